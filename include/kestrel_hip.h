@@ -1,0 +1,176 @@
+/* kestrel_hip.h -- C ABI of the MI355X (gfx950) Kestrel render backend.
+ *
+ * The drop-in boundary is the reference's `trait Backend` (src/backend.rs:22-58):
+ * set_gates / set_channels / set_params_variant / spawn / render / stats / name /
+ * timings, plus the constructor GpuSynth::new (src/gpu/mod.rs:276). Each entry
+ * point below names the reference interface it replaces. The reference's crate
+ * is rlib-only because its C ABI "is not implemented" (Cargo.toml:17-21); this
+ * header is that C ABI, layout-compatible with the reference's #[repr(C)]
+ * bytemuck Pod types, so a Rust FFI crate can bind it one-to-one.
+ *
+ * Conventions: every function returns 0 on success and a negative status on
+ * failure, never unwinds, and leaves a message for kst_last_error() on the
+ * calling thread. All slices are borrowed for the call only; the library copies
+ * what it keeps. One caller thread per backend at a time (the reference's
+ * `&mut self`).
+ */
+#ifndef KESTREL_HIP_H
+#define KESTREL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KST_OK 0
+#define KST_EINVAL (-1)   /* a validation failure; the message names the flag */
+#define KST_EDEVICE (-2)  /* a HIP runtime failure */
+#define KST_ESTATE (-3)   /* called out of order */
+
+/* One voice's birth record. src/voice.rs:22-57, 76 bytes, same field order. */
+typedef struct kst_spawn_cmd {
+    uint32_t phase_lo, phase_hi, step_lo, step_hi;
+    uint32_t smp_base, smp_len, loop_start, loop_end;
+    uint32_t flags, params, variant, region;
+    uint32_t gate_slot, ordinal, start_rel;
+    uint32_t note_id_lo, note_id_hi;
+    float gain_l, gain_r;
+} kst_spawn_cmd;
+
+/* Per-region DSP constants. src/bank.rs:166-190, 48 bytes. */
+typedef struct kst_region_params {
+    float attack_rate, attack_end, decay_coef, decay_target, sustain, release_coef;
+    float b0, b1, a1, a2;
+    uint32_t flags, pad;
+} kst_region_params;
+
+/* src/backend.rs:11-20 */
+typedef struct kst_block_stats {
+    uint64_t active_voices;
+    uint64_t stolen;
+    uint64_t dropped;
+    float peak;
+    uint32_t pad;
+} kst_block_stats;
+
+/* src/backend.rs:53-57, one (name, milliseconds) pair per pass group. */
+typedef struct kst_timing {
+    const char *name;
+    double ms;
+} kst_timing;
+
+/* The render-path subset of src/config.rs:164-343, in that file's units, plus
+ * the MI355X knobs (marked). Enum encodings follow the reference's #[repr(u32)]
+ * discriminants. */
+typedef struct kst_config {
+    uint32_t sample_rate;           /* Hz */
+    uint32_t block_frames;          /* frames per render() */
+    uint32_t gate_frames;           /* note-off / controller cadence */
+    uint32_t reduce_tile;           /* plain form only: frames per LDS reduction tile */
+    uint32_t workgroup_size;        /* plain form only: threads per render workgroup */
+    uint32_t max_render_workgroups; /* plain form only: partial-buffer width */
+    uint32_t max_voices;
+    uint32_t steal_rule;            /* 0 Oldest, 1 DropNew, 2 Quietest */
+    uint32_t admit_rule;            /* 0 Loudest, 1 Even */
+    uint32_t max_steal_percent;
+    uint32_t steal_fade_frames;
+    uint32_t sort_voices;           /* locality sort during compaction */
+    uint32_t interpolation;         /* 0 Nearest, 1 Linear, 2 Cubic */
+    uint32_t decay_curve;           /* 0 Exponential, 1 Linear */
+    uint32_t release_curve;         /* 0 Exponential, 1 Linear */
+    float env_floor;
+    uint32_t max_param_variants;
+    uint32_t kahan_reduce;          /* plain form: Kahan in the cross-workgroup reduce */
+    /* ---- MI355X knobs, no counterpart in the reference ---- */
+    uint32_t form;                  /* 0 plain (reference-structured render), 1 wave64 */
+    uint32_t compensated;           /* wave64 form: two-sum accumulation above the wave */
+    uint32_t waves_per_group;       /* wave64 form: waves per render workgroup (1..16) */
+    uint32_t groups_per_cu;         /* wave64 form: persistent render groups per CU */
+    uint32_t sort_digit_bits;       /* radix width of the locality sort, 1..8 */
+    int32_t device;                 /* HIP device ordinal */
+} kst_config;
+
+typedef struct kst_backend kst_backend;
+
+/* Fill `cfg` with the reference's defaults (src/config.rs:345-409) and the
+ * wave64 form. */
+void kst_default_config(kst_config *cfg);
+
+/* GpuSynth::new(&Config, Arc<Bank>) -- src/gpu/mod.rs:276-724. Only the bank's
+ * int16 pool and its params table cross to the device (src/gpu/mod.rs:346-388);
+ * the region count and longest sample size the locality key
+ * (SortKeyLayout::plan, src/gpu/mod.rs:76-105). */
+int kst_create(const kst_config *cfg, const int16_t *pool, size_t pool_len,
+               const kst_region_params *params, size_t n_params, uint32_t n_regions,
+               uint32_t max_sample_len, kst_backend **out);
+
+/* Drop for GpuSynth. */
+void kst_destroy(kst_backend *b);
+
+/* Backend::set_gates -- src/backend.rs:23-25. `n` == tiles * 2048. */
+int kst_set_gates(kst_backend *b, const uint32_t *rows, size_t n);
+
+/* Backend::set_channels -- src/backend.rs:27-33. `n` == tiles * 16 * 4. */
+int kst_set_channels(kst_backend *b, const uint32_t *rows, size_t n, int bend, int gain,
+                     int variant);
+
+/* Backend::set_params_variant -- src/backend.rs:35-39, src/gpu/mod.rs:972-987. */
+int kst_set_params_variant(kst_backend *b, uint32_t index, const kst_region_params *data,
+                           size_t n);
+
+/* Backend::spawn -- src/backend.rs:41-43. Steal and drop accounting is eager, as
+ * in src/cpu.rs:330-390: stats() after this call already counts this block's
+ * steals and drops. The device work happens in kst_render. */
+int kst_spawn(kst_backend *b, const kst_spawn_cmd *cmds, size_t n);
+
+/* Backend::render -- src/backend.rs:45-47. `n` == block_frames * 2, interleaved
+ * stereo. Advances every voice and compacts the pool; synchronous. */
+int kst_render(kst_backend *b, float *out, size_t n);
+
+/* Backend::stats -- src/backend.rs:49. */
+int kst_stats(const kst_backend *b, kst_block_stats *out);
+
+/* Backend::name -- src/backend.rs:51. */
+const char *kst_name(const kst_backend *b);
+
+/* Backend::timings -- src/backend.rs:53-57. Writes up to `cap` entries and
+ * returns how many there are. Empty until a block has rendered. */
+size_t kst_timings(const kst_backend *b, kst_timing *out, size_t cap);
+
+/* The message of the last failure on this thread (anyhow's error text). */
+const char *kst_last_error(void);
+
+/* ---- device-resident block program (no counterpart in the reference) ----
+ *
+ * The reference synchronises with the host once per block (src/gpu/mod.rs:
+ * 1246-1252). These stage the inputs of `nblocks` consecutive blocks in device
+ * memory -- gate rows, channel rows and flags, spawn commands -- and then run
+ * them back to back with the per-block steal/spawn arithmetic of
+ * src/gpu/mod.rs:1015-1040 evaluated on the device, so the host waits once.
+ * Results are identical to calling spawn/render per block. */
+int kst_stage_blocks(kst_backend *b, uint32_t nblocks, const uint32_t *gates,
+                     const uint32_t *chans, const uint32_t *chan_flags,
+                     const kst_spawn_cmd *cmds, const uint32_t *cmd_counts);
+/* Runs every staged block. use_graph != 0 replays the per-block launch chain
+ * from a captured hipGraph. Returns when the device is done. */
+int kst_run_staged(kst_backend *b, int use_graph);
+/* Copies out the staged run's output blocks (nblocks * block_frames * 2) and
+ * per-block stats (nblocks entries, as stats() would read after each render). */
+int kst_staged_results(kst_backend *b, float *out, kst_block_stats *stats);
+
+/* ---- test and checkpoint hooks (no counterpart in the reference) ---- */
+
+/* The live pool in the 24-field SoA layout of shaders/common.wgsl:15-44:
+ * field f of voice i at soa[f * stride + i]. Returns the live count. */
+int64_t kst_pool_export(kst_backend *b, uint32_t *soa, size_t stride);
+/* Replace the live pool with `n` voices in that layout. */
+int kst_pool_import(kst_backend *b, const uint32_t *soa, size_t n, size_t stride);
+/* Set the cumulative stolen/dropped counters (to resume from a snapshot). */
+int kst_set_counters(kst_backend *b, uint64_t stolen, uint64_t dropped);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KESTREL_HIP_H */

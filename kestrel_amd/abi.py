@@ -1,0 +1,116 @@
+"""ctypes binding of the C ABI in include/kestrel_hip.h (libkestrel_hip.so).
+
+The library is built in-tree by kestrel_amd/build.py. Loading it needs the HIP
+runtime; calling anything that touches the device needs a GPU. There is no
+fallback: if the library is missing or cannot load, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .config import KstConfig
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libkestrel_hip.so")
+HEADER_PATH = os.path.join(os.path.dirname(HERE), "include", "kestrel_hip.h")
+
+# kst_spawn_cmd, src/voice.rs:22-57 (76 B)
+SPAWN_DTYPE = np.dtype([
+    ("phase_lo", "<u4"), ("phase_hi", "<u4"), ("step_lo", "<u4"), ("step_hi", "<u4"),
+    ("smp_base", "<u4"), ("smp_len", "<u4"), ("loop_start", "<u4"), ("loop_end", "<u4"),
+    ("flags", "<u4"), ("params", "<u4"), ("variant", "<u4"), ("region", "<u4"),
+    ("gate_slot", "<u4"), ("ordinal", "<u4"), ("start_rel", "<u4"),
+    ("note_id_lo", "<u4"), ("note_id_hi", "<u4"), ("gain_l", "<f4"), ("gain_r", "<f4"),
+])
+assert SPAWN_DTYPE.itemsize == 76
+
+# kst_region_params, src/bank.rs:166-190 (48 B)
+PARAMS_DTYPE = np.dtype([
+    ("attack_rate", "<f4"), ("attack_end", "<f4"), ("decay_coef", "<f4"),
+    ("decay_target", "<f4"), ("sustain", "<f4"), ("release_coef", "<f4"),
+    ("b0", "<f4"), ("b1", "<f4"), ("a1", "<f4"), ("a2", "<f4"),
+    ("flags", "<u4"), ("pad", "<u4"),
+])
+assert PARAMS_DTYPE.itemsize == 48
+
+VOICE_FIELDS = 24
+# shaders/common.wgsl:15-43
+FIELDS = ("phase_lo", "phase_hi", "step_lo", "step_hi", "smp_base", "smp_len", "loop_start",
+          "loop_end", "flags", "env_stage", "env_level", "gain_l", "gain_r", "filt_z1",
+          "filt_z2", "params", "region", "gate_slot", "ordinal", "start_rel", "note_lo",
+          "note_hi", "born_variant", "stop_rel")
+FIELD = {n: i for i, n in enumerate(FIELDS)}
+
+
+class KstBlockStats(C.Structure):
+    _fields_ = [("active_voices", C.c_uint64), ("stolen", C.c_uint64),
+                ("dropped", C.c_uint64), ("peak", C.c_float), ("pad", C.c_uint32)]
+
+
+class KstTiming(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("ms", C.c_double)]
+
+
+# Every symbol the header declares, with its ctypes signature.
+P, u32, u64, i32, i64, sz = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int, C.c_int64, C.c_size_t
+SIGNATURES = {
+    "kst_default_config": (None, [P]),
+    "kst_create": (i32, [P, P, sz, P, sz, u32, u32, P]),
+    "kst_destroy": (None, [P]),
+    "kst_set_gates": (i32, [P, P, sz]),
+    "kst_set_channels": (i32, [P, P, sz, i32, i32, i32]),
+    "kst_set_params_variant": (i32, [P, u32, P, sz]),
+    "kst_spawn": (i32, [P, P, sz]),
+    "kst_render": (i32, [P, P, sz]),
+    "kst_stats": (i32, [P, P]),
+    "kst_name": (C.c_char_p, [P]),
+    "kst_timings": (sz, [P, P, sz]),
+    "kst_last_error": (C.c_char_p, []),
+    "kst_stage_blocks": (i32, [P, u32, P, P, P, P, P]),
+    "kst_run_staged": (i32, [P, i32]),
+    "kst_staged_results": (i32, [P, P, P]),
+    "kst_pool_export": (i64, [P, P, sz]),
+    "kst_pool_import": (i32, [P, P, sz, sz]),
+    "kst_set_counters": (i32, [P, u64, u64]),
+}
+
+_lib = None
+
+
+class BackendError(RuntimeError):
+    pass
+
+
+def load() -> C.CDLL:
+    """Load the in-tree library. Raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise BackendError(f"{LIB_PATH} is not built; run kestrel_amd/build.py "
+                               "(or __graft_entry__.build())")
+        lib = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        msg = load().kst_last_error().decode(errors="replace")
+        raise BackendError(f"kestrel_hip error {rc}: {msg}")
+
+
+def ptr(a: np.ndarray) -> C.c_void_p:
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def default_config() -> KstConfig:
+    k = KstConfig()
+    load().kst_default_config(C.byref(k))
+    return k

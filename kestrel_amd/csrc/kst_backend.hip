@@ -1,0 +1,826 @@
+// kst_backend.hip -- host orchestration of the gfx950 render backend behind the
+// C ABI of include/kestrel_hip.h.
+//
+// One backend = one pool on one device, one HIP stream, and the per-block
+// launch chain of src/gpu/mod.rs:1014-1289 rebuilt around a device-resident
+// block state: k_begin evaluates the block's steal/spawn arithmetic on the
+// device from the live count the previous block left there, so the same chain
+// serves the synchronous Backend::render and runs of staged blocks (optionally
+// replayed from a captured hipGraph) without the host in between.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kst_device.h"
+
+using namespace kst;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return fail(KST_EDEVICE, "%s failed: %s", #expr, hipGetErrorString(e_));   \
+    } while (0)
+
+// The reference's Config::max_steal / pool_slots / steal_span
+// (src/config.rs:534-552); both sides must use exactly this arithmetic.
+uint32_t cfg_max_steal(const kst_config &c) {
+    uint64_t n = (uint64_t)c.max_voices * c.max_steal_percent / 100;
+    uint32_t v = (uint32_t)n;
+    return v > 1 ? v : 1;
+}
+uint32_t cfg_pool_slots(const kst_config &c) {
+    uint64_t v = (uint64_t)c.max_voices + cfg_max_steal(c);
+    return v > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)v;
+}
+uint32_t cfg_steal_span(const kst_config &c) {
+    uint32_t s = c.block_frames > c.steal_fade_frames ? c.block_frames - c.steal_fade_frames : 0;
+    return s > 1 ? s : 1;
+}
+
+bool is_pow2(uint32_t x) { return x && !(x & (x - 1)); }
+
+// src/config.rs:412-525, the render-path subset, plus the device's own limits.
+int validate(const kst_config &c) {
+    if (c.sample_rate < 8000 || c.sample_rate > 768000)
+        return fail(KST_EINVAL, "sample_rate %u out of range", c.sample_rate);
+    if (c.block_frames == 0 || c.reduce_tile == 0 || c.block_frames % c.reduce_tile != 0)
+        return fail(KST_EINVAL, "block_frames (%u) must be a non-zero multiple of reduce_tile (%u)",
+                    c.block_frames, c.reduce_tile);
+    if (!is_pow2(c.workgroup_size) || c.workgroup_size < 32 || c.workgroup_size > 1024)
+        return fail(KST_EINVAL, "workgroup_size %u must be a power of two in [32, 1024]",
+                    c.workgroup_size);
+    if (!is_pow2(c.reduce_tile))
+        return fail(KST_EINVAL, "reduce_tile %u must be a power of two", c.reduce_tile);
+    if (c.gate_frames == 0 || c.gate_frames % c.reduce_tile != 0 ||
+        c.block_frames % c.gate_frames != 0)
+        return fail(KST_EINVAL,
+                    "gate_frames (%u) must be a multiple of reduce_tile (%u) and divide "
+                    "block_frames (%u)",
+                    c.gate_frames, c.reduce_tile, c.block_frames);
+    uint64_t shared = (uint64_t)c.workgroup_size * (c.reduce_tile * 2ull + 1) * 4;
+    if (shared > 65536)
+        return fail(KST_EINVAL,
+                    "workgroup_size %u with reduce_tile %u needs %llu bytes of workgroup "
+                    "storage, over the 64 KiB a workgroup can have",
+                    c.workgroup_size, c.reduce_tile, (unsigned long long)shared);
+    if (c.workgroup_size < c.reduce_tile * 2)
+        return fail(KST_EINVAL, "workgroup_size %u must be at least twice reduce_tile %u",
+                    c.workgroup_size, c.reduce_tile);
+    if (c.max_voices == 0) return fail(KST_EINVAL, "max_voices must be non-zero");
+    if (c.steal_fade_frames == 0 || c.steal_fade_frames >= c.block_frames)
+        return fail(KST_EINVAL, "steal_fade_frames %u must be in 1..block_frames (%u)",
+                    c.steal_fade_frames, c.block_frames);
+    if (c.max_steal_percent == 0 || c.max_steal_percent > 100)
+        return fail(KST_EINVAL, "max_steal_percent %u must be in 1..=100", c.max_steal_percent);
+    if (c.block_frames > 65535)
+        return fail(KST_EINVAL, "block_frames %u must be at most 65535", c.block_frames);
+    if (c.steal_rule > 2) return fail(KST_EINVAL, "steal_rule %u unknown", c.steal_rule);
+    if (c.admit_rule > 1) return fail(KST_EINVAL, "admit_rule %u unknown", c.admit_rule);
+    if (c.interpolation > 2) return fail(KST_EINVAL, "interpolation %u unknown", c.interpolation);
+    if (c.decay_curve > 1 || c.release_curve > 1)
+        return fail(KST_EINVAL, "envelope curve must be 0 (exponential) or 1 (linear)");
+    if (c.max_param_variants == 0) return fail(KST_EINVAL, "max_param_variants must be non-zero");
+    if (c.waves_per_group == 0 || c.waves_per_group > 16)
+        return fail(KST_EINVAL, "waves_per_group %u must be in 1..=16", c.waves_per_group);
+    if (c.groups_per_cu == 0) return fail(KST_EINVAL, "groups_per_cu must be non-zero");
+    if (c.sort_digit_bits == 0 || c.sort_digit_bits > 8)
+        return fail(KST_EINVAL, "sort_digit_bits %u must be in 1..=8", c.sort_digit_bits);
+    // 32-bit SoA word offsets: field 23 of the last slot must stay below 2^32.
+    if ((uint64_t)cfg_pool_slots(c) * VOICE_FIELDS >= (1ull << 32))
+        return fail(KST_EINVAL,
+                    "max_voices %u allocates %u pool slots; the 24-field pool is addressed "
+                    "with 32-bit word offsets, so cap --max-voices at %llu",
+                    c.max_voices, cfg_pool_slots(c),
+                    (unsigned long long)(((1ull << 32) / VOICE_FIELDS - 1) * 100 /
+                                         (100 + c.max_steal_percent)));
+    return KST_OK;
+}
+
+struct SortPlan {
+    uint32_t bits, region_shift, stage_shift, phase_shift, phase_mask, dead_region;
+};
+
+// src/gpu/mod.rs:76-105
+SortPlan plan_sort(uint32_t n_regions, uint32_t max_len) {
+    auto bitlen = [](uint32_t x) -> uint32_t { return x == 0 ? 0 : 32 - __builtin_clz(x); };
+    SortPlan p;
+    p.dead_region = n_regions > 1 ? n_regions : 1;
+    uint32_t region_bits = bitlen(p.dead_region);
+    uint32_t stage_bits = 3;
+    uint32_t len_bits = bitlen(max_len > 1 ? max_len : 1);
+    uint32_t avail = 32 > region_bits + stage_bits ? 32 - (region_bits + stage_bits) : 0;
+    uint32_t phase_bits = std::min(avail, len_bits);
+    p.bits = region_bits + stage_bits + phase_bits;
+    p.region_shift = phase_bits + stage_bits;
+    p.stage_shift = phase_bits;
+    p.phase_shift = len_bits > phase_bits ? len_bits - phase_bits : 0;
+    p.phase_mask = phase_bits >= 32 ? 0xFFFFFFFFu : ((1u << phase_bits) - 1);
+    return p;
+}
+
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    hipError_t alloc(size_t count) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = count;
+        return hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+const char *PASS_NAMES[5] = {"steal", "spawn", "render", "reduce", "compact"};
+
+}  // namespace
+
+struct kst_backend {
+    kst_config cfg;
+    int device = 0;
+    uint32_t cus = 256;
+    hipStream_t stream = nullptr;
+    bool fast = true;  // wave64 form in use
+
+    uint32_t slots = 0, ppv = 0, tiles = 0, gate_words = 0, chan_words = 0;
+    uint32_t max_steal = 0;
+    SortPlan sort{};
+    uint32_t sort_passes = 1, digit_bits = 1, tiles_max = 1;
+    uint32_t max_groups = 1;  // partial-buffer rows
+
+    DevBuf<uint32_t> pools[2];
+    DevBuf<int16_t> bank;
+    uint32_t bank_len = 0;
+    DevBuf<kst_region_params> params;
+    DevBuf<float> partials;
+    DevBuf<uint32_t> keys[2], idx[2], sort_hist, sort_tot, sel_hist;
+    DevBuf<DevState> st;
+
+    // Staged inputs (one slot for the per-block API, nblocks for staged runs).
+    uint32_t stage_cap = 0, staged = 0;
+    DevBuf<uint32_t> g_gates, g_chans, g_chan_flags, g_cmd_counts, g_cmd_offs;
+    DevBuf<kst_spawn_cmd> g_cmds;
+    DevBuf<float> g_out;
+    DevBuf<BlockResult> g_results;
+
+    // Host side of the per-block API.
+    std::vector<uint32_t> h_gates, h_chans;
+    uint32_t h_chan_flags = 0;
+    std::vector<kst_spawn_cmd> pending;
+    uint32_t *pin_small = nullptr;  // pinned staging for per-block uploads
+    size_t pin_cmds_cap = 0;
+    kst_spawn_cmd *pin_cmds = nullptr;
+    float *pin_out = nullptr;
+    BlockResult *pin_result = nullptr;
+
+    // Host mirror, exact: every change is host-decided or read back.
+    uint32_t live = 0, parity = 0;
+    uint64_t stolen = 0, dropped = 0;
+    float peak = 0.0f;
+
+    bool profile = false;
+    hipEvent_t ev[6] = {};
+    std::vector<kst_timing> timings;
+
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+
+    ~kst_backend() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
+        for (auto &p : pools) p.release();
+        bank.release();
+        params.release();
+        partials.release();
+        for (auto &k : keys) k.release();
+        for (auto &k : idx) k.release();
+        sort_hist.release();
+        sort_tot.release();
+        sel_hist.release();
+        st.release();
+        g_gates.release();
+        g_chans.release();
+        g_chan_flags.release();
+        g_cmd_counts.release();
+        g_cmd_offs.release();
+        g_cmds.release();
+        g_out.release();
+        g_results.release();
+        if (pin_small) (void)hipHostFree(pin_small);
+        if (pin_cmds) (void)hipHostFree(pin_cmds);
+        if (pin_out) (void)hipHostFree(pin_out);
+        if (pin_result) (void)hipHostFree(pin_result);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    int ensure_stage(uint32_t nblocks, size_t total_cmds) {
+        if (nblocks > stage_cap) {
+            HIPCHK(g_gates.alloc((size_t)nblocks * gate_words));
+            HIPCHK(g_chans.alloc((size_t)nblocks * chan_words));
+            HIPCHK(g_chan_flags.alloc(nblocks));
+            HIPCHK(g_cmd_counts.alloc(nblocks));
+            HIPCHK(g_cmd_offs.alloc(nblocks));
+            HIPCHK(g_out.alloc((size_t)nblocks * 2 * cfg.block_frames));
+            HIPCHK(g_results.alloc(nblocks));
+            stage_cap = nblocks;
+            drop_graph();
+        }
+        if (total_cmds > g_cmds.n || !g_cmds.p) {
+            size_t n = std::max<size_t>(total_cmds, 1024);
+            HIPCHK(g_cmds.alloc(n));
+            drop_graph();
+        }
+        return KST_OK;
+    }
+
+    void drop_graph() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph = nullptr;
+    }
+
+    uint32_t per_voice_grid(uint32_t n) const {
+        uint32_t g = (n + 255) / 256;
+        return std::max(1u, std::min(g, cus * 8));
+    }
+    uint32_t sort_grid(uint32_t n) const {
+        uint32_t t = (n + SORT_TILE - 1) / SORT_TILE;
+        return std::max(1u, std::min(t, cus * 4));
+    }
+    uint32_t render_groups_for(uint32_t n) const {
+        if (fast) {
+            uint32_t waves = (n + 63) / 64;
+            uint32_t g = (waves + cfg.waves_per_group - 1) / cfg.waves_per_group;
+            return std::max(1u, std::min(g, max_groups));
+        }
+        uint32_t cap = std::min(std::max(cfg.max_render_workgroups, 1u), 65535u);
+        uint32_t g = (n + cfg.workgroup_size - 1) / cfg.workgroup_size;
+        return std::max(1u, std::min(g, std::min(cap, max_groups)));
+    }
+
+    Args args(uint32_t render_groups) const {
+        Args a{};
+        a.st = st.p;
+        a.pools[0] = pools[0].p;
+        a.pools[1] = pools[1].p;
+        a.cap = slots;
+        a.bank_len = bank_len;
+        a.bank = bank.p;
+        a.params = params.p;
+        a.ppv = ppv;
+        a.max_variants = cfg.max_param_variants;
+        a.gates = g_gates.p;
+        a.chans = g_chans.p;
+        a.chan_flags = g_chan_flags.p;
+        a.cmds = g_cmds.p;
+        a.cmd_counts = g_cmd_counts.p;
+        a.cmd_offs = g_cmd_offs.p;
+        a.gate_words = gate_words;
+        a.chan_words = chan_words;
+        a.out = g_out.p;
+        a.results = g_results.p;
+        a.F = cfg.block_frames;
+        a.gate_frames = cfg.gate_frames;
+        a.max_voices = cfg.max_voices;
+        a.max_steal = max_steal;
+        a.steal_span = cfg_steal_span(cfg);
+        a.fade = cfg.steal_fade_frames;
+        a.steal_rule = cfg.steal_rule;
+        a.admit_rule = cfg.admit_rule;
+        a.interp = cfg.interpolation;
+        a.exp_decay = cfg.decay_curve == 0;
+        a.exp_release = cfg.release_curve == 0;
+        a.env_floor = cfg.env_floor;
+        a.partials = partials.p;
+        a.render_groups = render_groups;
+        a.waves_per_group = cfg.waves_per_group;
+        a.plain_tile = cfg.reduce_tile;
+        a.plain_wg = cfg.workgroup_size;
+        a.kahan = cfg.kahan_reduce;
+        for (int k = 0; k < 2; ++k) {
+            a.keys[k] = keys[k].p;
+            a.idx[k] = idx[k].p;
+        }
+        a.sort_hist = sort_hist.p;
+        a.sort_tot = sort_tot.p;
+        a.sort_on = cfg.sort_voices;
+        a.sort_passes = sort_passes;
+        a.digit_bits = digit_bits;
+        a.tiles_max = tiles_max;
+        a.region_shift = sort.region_shift;
+        a.stage_shift = sort.stage_shift;
+        a.phase_shift = sort.phase_shift;
+        a.phase_mask = sort.phase_mask;
+        a.dead_region = sort.dead_region;
+        a.sel_hist = sel_hist.p;
+        return a;
+    }
+
+    // The per-block chain. `n_live`/`n_total` size the grids; the kernels read
+    // the true counts from the device state, so capacity-sized grids (staged
+    // runs) are as correct as exact ones (per-block calls).
+    void launch_block(uint32_t n_live, uint32_t n_total, bool may_steal, bool spawn, bool chan,
+                      bool timed) {
+        const uint32_t rg = render_groups_for(n_total);
+        Args a = args(rg);
+        hipStream_t s = stream;
+        if (timed) (void)hipEventRecord(ev[0], s);
+        launch_begin(a, s);
+        if (may_steal) launch_select(a, per_voice_grid(n_live), s);
+        if (timed) (void)hipEventRecord(ev[1], s);
+        if (spawn) launch_spawn(a, per_voice_grid(n_total), s);
+        if (timed) (void)hipEventRecord(ev[2], s);
+        if (fast)
+            launch_render_fast(a, chan, cfg.compensated != 0, s);
+        else
+            launch_render_plain(a, chan, s);
+        if (timed) (void)hipEventRecord(ev[3], s);
+        if (fast)
+            launch_reduce_fast(a, cfg.compensated != 0, s);
+        else
+            launch_reduce_plain(a, s);
+        if (timed) (void)hipEventRecord(ev[4], s);
+        launch_sort(a, sort_grid(n_total), s);
+        launch_end(a, s);
+        if (timed) (void)hipEventRecord(ev[5], s);
+    }
+
+    void read_timings() {
+        timings.clear();
+        for (int i = 0; i < 5; ++i) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) != hipSuccess) ms = 0.0f;
+            timings.push_back({PASS_NAMES[i], (double)ms});
+        }
+    }
+};
+
+extern "C" {
+
+void kst_default_config(kst_config *c) {
+    std::memset(c, 0, sizeof *c);
+    // src/config.rs:345-409
+    c->sample_rate = 48000;
+    c->block_frames = 4096;
+    c->gate_frames = 32;
+    c->reduce_tile = 4;
+    c->workgroup_size = 256;
+    c->max_render_workgroups = 2048;
+    c->max_voices = 1u << 20;
+    c->steal_rule = 2;  // Quietest
+    c->admit_rule = 0;  // Loudest
+    c->max_steal_percent = 25;
+    c->steal_fade_frames = 96;
+    c->sort_voices = 1;
+    c->interpolation = 1;
+    c->decay_curve = 0;
+    c->release_curve = 0;
+    c->env_floor = 1.0e-5f;
+    c->max_param_variants = 32;
+    c->kahan_reduce = 0;
+    c->form = 1;
+    c->compensated = 0;
+    c->waves_per_group = 4;
+    c->groups_per_cu = 3;
+    c->sort_digit_bits = 8;
+    c->device = 0;
+}
+
+const char *kst_last_error(void) { return g_last_error.c_str(); }
+
+int kst_create(const kst_config *cfg, const int16_t *pool, size_t pool_len,
+               const kst_region_params *params_in, size_t n_params, uint32_t n_regions,
+               uint32_t max_sample_len, kst_backend **out) {
+    if (!cfg || !out) return fail(KST_EINVAL, "null argument");
+    *out = nullptr;
+    int rc = validate(*cfg);
+    if (rc) return rc;
+    auto *b = new kst_backend();
+    b->cfg = *cfg;
+    b->device = cfg->device;
+    auto cleanup = [&](int code) {
+        delete b;
+        return code;
+    };
+    hipError_t e = hipSetDevice(b->device);
+    if (e != hipSuccess)
+        return cleanup(fail(KST_EDEVICE, "hipSetDevice(%d): %s", b->device, hipGetErrorString(e)));
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device) == hipSuccess &&
+        cus > 0)
+        b->cus = (uint32_t)cus;
+    if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess)
+        return cleanup(fail(KST_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(e)));
+
+    const kst_config &c = b->cfg;
+    b->fast = c.form == 1 && fast_render_supported(c.block_frames, c.gate_frames, c.interpolation);
+    b->slots = cfg_pool_slots(c);
+    b->max_steal = cfg_max_steal(c);
+    b->tiles = c.block_frames / c.gate_frames;
+    b->gate_words = b->tiles * GATE_SLOTS;
+    b->chan_words = b->tiles * BEND_CHANNELS * CHAN_FIELDS;
+    b->sort = plan_sort(n_regions, max_sample_len);
+    if (c.sort_voices) {
+        b->digit_bits = c.sort_digit_bits;
+        b->sort_passes = (b->sort.bits + b->digit_bits - 1) / b->digit_bits;
+        if (b->sort_passes == 0) b->sort_passes = 1;
+    } else {
+        b->digit_bits = 1;
+        b->sort_passes = 1;
+    }
+    b->tiles_max = (b->slots + SORT_TILE - 1) / SORT_TILE;
+    if (b->fast) {
+        uint32_t waves = (b->slots + 63) / 64;
+        uint32_t g = (waves + c.waves_per_group - 1) / c.waves_per_group;
+        b->max_groups = std::max(1u, std::min(g, b->cus * c.groups_per_cu));
+        size_t lds = fast_render_lds_bytes(c.block_frames, c.waves_per_group, c.compensated != 0);
+        if (lds > 160 * 1024)
+            return cleanup(fail(KST_EINVAL,
+                                "block_frames %u with %u waves per group needs %zu bytes of LDS, "
+                                "over the 160 KiB of a CU; lower waves_per_group",
+                                c.block_frames, c.waves_per_group, lds));
+    } else {
+        uint32_t cap = std::min(std::max(c.max_render_workgroups, 1u), 65535u);
+        b->max_groups = std::max(1u, std::min(cap, (b->slots + c.workgroup_size - 1) / c.workgroup_size));
+    }
+
+    // ---- static data: the bank's pool, padded with zero guard frames so
+    // every neighbour read of the last frame stays inside the buffer.
+    std::vector<int16_t> padded(pool, pool + pool_len);
+    padded.resize(pool_len + 8 + (pool_len & 1), 0);
+    b->bank_len = (uint32_t)std::min<size_t>(pool_len, 0xFFFFFFFFu);
+    if (b->bank.alloc(padded.size()) != hipSuccess)
+        return cleanup(fail(KST_EDEVICE, "allocating the %zu-sample pool", padded.size()));
+    if (hipMemcpy(b->bank.p, padded.data(), padded.size() * 2, hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(fail(KST_EDEVICE, "uploading the sample pool"));
+
+    // src/gpu/mod.rs:357-382: an empty table gets one neutral entry.
+    std::vector<kst_region_params> ptab(params_in, params_in + n_params);
+    if (ptab.empty()) ptab.push_back({1, 1, 1, 1, 1, 0, 1, 0, 0, 0, 0, 0});
+    b->ppv = (uint32_t)ptab.size();
+    // Every variant slot starts as the bank's own table (what src/cpu.rs:228-238
+    // falls back to for a variant never uploaded).
+    std::vector<kst_region_params> all((size_t)b->ppv * c.max_param_variants);
+    for (uint32_t v = 0; v < c.max_param_variants; ++v)
+        std::copy(ptab.begin(), ptab.end(), all.begin() + (size_t)v * b->ppv);
+    if (b->params.alloc(all.size()) != hipSuccess ||
+        hipMemcpy(b->params.p, all.data(), all.size() * sizeof(kst_region_params),
+                  hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(fail(KST_EDEVICE, "uploading the params table"));
+
+    // ---- per-block buffers ----
+    const size_t pool_words = (size_t)b->slots * VOICE_FIELDS;
+    const size_t two_f = 2 * (size_t)c.block_frames;
+    for (int k = 0; k < 2; ++k) {
+        if (b->pools[k].alloc(pool_words) != hipSuccess)
+            return cleanup(fail(KST_EDEVICE,
+                                "allocating a %.2f GiB voice pool buffer for %u slots",
+                                pool_words * 4.0 / (1 << 30), b->slots));
+        if (b->keys[k].alloc(b->slots) != hipSuccess || b->idx[k].alloc(b->slots) != hipSuccess)
+            return cleanup(fail(KST_EDEVICE, "allocating the sort buffers"));
+    }
+    const size_t prow = b->fast ? two_f * (c.compensated ? 2 : 1) : two_f;
+    if (b->partials.alloc(prow * b->max_groups) != hipSuccess)
+        return cleanup(fail(KST_EDEVICE, "allocating the partial sums"));
+    const uint32_t radix = 1u << b->digit_bits;
+    if (b->sort_hist.alloc((size_t)b->sort_passes * radix * b->tiles_max) != hipSuccess ||
+        b->sort_tot.alloc((size_t)b->sort_passes * radix) != hipSuccess ||
+        b->sel_hist.alloc(SEL_ROUNDS * 256) != hipSuccess || b->st.alloc(1) != hipSuccess)
+        return cleanup(fail(KST_EDEVICE, "allocating the select/sort scratch"));
+    DevState zero{};
+    if (hipMemcpy(b->st.p, &zero, sizeof zero, hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(fail(KST_EDEVICE, "initialising the device state"));
+    if (b->ensure_stage(1, 1024) != KST_OK) return cleanup(KST_EDEVICE);
+
+    b->h_gates.assign(b->gate_words, 0);
+    b->h_chans.assign(b->chan_words, 0);
+    if (hipHostMalloc(&b->pin_small, 4096 * sizeof(uint32_t)) != hipSuccess ||
+        hipHostMalloc(&b->pin_out, two_f * sizeof(float)) != hipSuccess ||
+        hipHostMalloc(&b->pin_result, sizeof(BlockResult)) != hipSuccess)
+        return cleanup(fail(KST_EDEVICE, "allocating pinned staging"));
+    for (auto &x : b->ev)
+        if (hipEventCreate(&x) != hipSuccess)
+            return cleanup(fail(KST_EDEVICE, "creating timing events"));
+    b->profile = getenv("KST_PROFILE") != nullptr;
+    *out = b;
+    return KST_OK;
+}
+
+void kst_destroy(kst_backend *b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    (void)hipStreamSynchronize(b->stream);
+    delete b;
+}
+
+int kst_set_gates(kst_backend *b, const uint32_t *rows, size_t n) {
+    if (!b || (!rows && n)) return fail(KST_EINVAL, "null argument");
+    if (n != b->gate_words)
+        return fail(KST_EINVAL, "gate rows are %zu words, expected %u", n, b->gate_words);
+    std::memcpy(b->h_gates.data(), rows, n * 4);
+    return KST_OK;
+}
+
+int kst_set_channels(kst_backend *b, const uint32_t *rows, size_t n, int bend, int gain,
+                     int variant) {
+    if (!b || (!rows && n)) return fail(KST_EINVAL, "null argument");
+    if (n != b->chan_words)
+        return fail(KST_EINVAL, "channel rows are %zu words, expected %u", n, b->chan_words);
+    std::memcpy(b->h_chans.data(), rows, n * 4);
+    b->h_chan_flags = (bend ? CHAN_ACTIVE_BEND : 0) | (gain ? CHAN_ACTIVE_GAIN : 0) |
+                      (variant ? CHAN_ACTIVE_VARIANT : 0);
+    return KST_OK;
+}
+
+int kst_set_params_variant(kst_backend *b, uint32_t index, const kst_region_params *data,
+                           size_t n) {
+    if (!b || (!data && n)) return fail(KST_EINVAL, "null argument");
+    // src/gpu/mod.rs:972-987
+    if (n != b->ppv)
+        return fail(KST_EINVAL, "params variant %u has %zu entries, expected %u", index, n, b->ppv);
+    if (index >= b->cfg.max_param_variants)
+        return fail(KST_EINVAL, "params variant %u is past the configured maximum", index);
+    (void)hipSetDevice(b->device);
+    HIPCHK(hipMemcpy(b->params.p + (size_t)index * b->ppv, data, n * sizeof(kst_region_params),
+                     hipMemcpyHostToDevice));
+    return KST_OK;
+}
+
+int kst_spawn(kst_backend *b, const kst_spawn_cmd *cmds, size_t n) {
+    if (!b || (!cmds && n)) return fail(KST_EINVAL, "null argument");
+    if (n > 0xFFFFFFFFull) return fail(KST_EINVAL, "%zu spawn commands in one block", n);
+    b->pending.assign(cmds, cmds + n);
+    // Eager accounting, src/cpu.rs:330-390; the device repeats this arithmetic
+    // in k_begin from the same inputs.
+    const uint64_t cap = b->cfg.max_voices, live = b->live, raw = n;
+    const uint64_t want = std::min<uint64_t>(raw, cap);
+    uint64_t need = 0;
+    if (live + want > cap && b->cfg.steal_rule != STEAL_DROP_NEW)
+        need = std::min<uint64_t>(std::min<uint64_t>(live + want - cap, live), b->max_steal);
+    const uint64_t take = std::min<uint64_t>(want, cap - (live - need));
+    b->stolen += need;
+    b->dropped += raw - take;
+    return KST_OK;
+}
+
+int kst_render(kst_backend *b, float *out, size_t n) {
+    if (!b || !out) return fail(KST_EINVAL, "null argument");
+    const size_t two_f = 2 * (size_t)b->cfg.block_frames;
+    if (n != two_f) return fail(KST_EINVAL, "output block is %zu samples, expected %zu", n, two_f);
+    HIPCHK(hipSetDevice(b->device));
+    const uint32_t ncmd = (uint32_t)b->pending.size();
+    int rc = b->ensure_stage(1, ncmd);
+    if (rc) return rc;
+    if (ncmd > b->pin_cmds_cap) {
+        if (b->pin_cmds) (void)hipHostFree(b->pin_cmds);
+        b->pin_cmds = nullptr;
+        size_t cap = std::max<size_t>(ncmd, 1024);
+        HIPCHK(hipHostMalloc(&b->pin_cmds, cap * sizeof(kst_spawn_cmd)));
+        b->pin_cmds_cap = cap;
+    }
+    hipStream_t s = b->stream;
+    // Block slot 0: gates, channels, flags, commands, and the block index.
+    HIPCHK(hipMemcpyAsync(b->g_gates.p, b->h_gates.data(), b->gate_words * 4,
+                          hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(b->g_chans.p, b->h_chans.data(), b->chan_words * 4,
+                          hipMemcpyHostToDevice, s));
+    if (ncmd) {
+        std::memcpy(b->pin_cmds, b->pending.data(), (size_t)ncmd * sizeof(kst_spawn_cmd));
+        HIPCHK(hipMemcpyAsync(b->g_cmds.p, b->pin_cmds, (size_t)ncmd * sizeof(kst_spawn_cmd),
+                              hipMemcpyHostToDevice, s));
+    }
+    b->pin_small[0] = b->h_chan_flags;
+    b->pin_small[1] = ncmd;
+    b->pin_small[2] = 0;  // command offset
+    b->pin_small[3] = 0;  // block index
+    HIPCHK(hipMemcpyAsync(b->g_chan_flags.p, &b->pin_small[0], 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(b->g_cmd_counts.p, &b->pin_small[1], 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(b->g_cmd_offs.p, &b->pin_small[2], 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(&b->st.p->block, &b->pin_small[3], 4, hipMemcpyHostToDevice, s));
+
+    // Host copy of k_begin's arithmetic, for grid sizes only.
+    const uint64_t cap = b->cfg.max_voices, live = b->live;
+    const uint64_t want = std::min<uint64_t>(ncmd, cap);
+    uint64_t need = 0;
+    if (live + want > cap && b->cfg.steal_rule != STEAL_DROP_NEW)
+        need = std::min<uint64_t>(std::min<uint64_t>(live + want - cap, live), b->max_steal);
+    const uint64_t take = std::min<uint64_t>(want, cap - (live - need));
+    const uint32_t total = (uint32_t)(live + take);
+    b->launch_block((uint32_t)live, total, need > 0, take > 0, b->h_chan_flags != 0, b->profile);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(b->pin_out, b->g_out.p, two_f * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(b->pin_result, b->g_results.p, sizeof(BlockResult),
+                          hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    std::memcpy(out, b->pin_out, two_f * 4);
+    const BlockResult &r = *b->pin_result;
+    b->live = (uint32_t)r.active_voices;
+    b->parity ^= 1;
+    float pk;
+    std::memcpy(&pk, &r.peak_bits, 4);
+    b->peak = pk;
+    if (r.stolen != b->stolen || r.dropped != b->dropped)
+        return fail(KST_ESTATE, "device steal/drop counters (%llu/%llu) disagree with the host's "
+                                "(%llu/%llu)",
+                    (unsigned long long)r.stolen, (unsigned long long)r.dropped,
+                    (unsigned long long)b->stolen, (unsigned long long)b->dropped);
+    if (b->profile) b->read_timings();
+    b->pending.clear();
+    return KST_OK;
+}
+
+int kst_stats(const kst_backend *b, kst_block_stats *out) {
+    if (!b || !out) return fail(KST_EINVAL, "null argument");
+    // After spawn and before render the CPU reference counts the appended
+    // voices (src/cpu.rs:641-648); so does this.
+    uint64_t active = b->live;
+    if (!b->pending.empty()) {
+        const uint64_t cap = b->cfg.max_voices, live = b->live;
+        const uint64_t want = std::min<uint64_t>(b->pending.size(), cap);
+        uint64_t need = 0;
+        if (live + want > cap && b->cfg.steal_rule != STEAL_DROP_NEW)
+            need = std::min<uint64_t>(std::min<uint64_t>(live + want - cap, live), b->max_steal);
+        active = live + std::min<uint64_t>(want, cap - (live - need));
+    }
+    out->active_voices = active;
+    out->stolen = b->stolen;
+    out->dropped = b->dropped;
+    out->peak = b->peak;
+    out->pad = 0;
+    return KST_OK;
+}
+
+const char *kst_name(const kst_backend *b) {
+    if (!b) return "";
+    return b->fast ? (b->cfg.compensated ? "hip-wave64-compensated" : "hip-wave64") : "hip-plain";
+}
+
+size_t kst_timings(const kst_backend *b, kst_timing *out, size_t cap) {
+    if (!b) return 0;
+    size_t n = b->timings.size();
+    for (size_t i = 0; i < n && i < cap; ++i) out[i] = b->timings[i];
+    return n;
+}
+
+int kst_stage_blocks(kst_backend *b, uint32_t nblocks, const uint32_t *gates,
+                     const uint32_t *chans, const uint32_t *chan_flags,
+                     const kst_spawn_cmd *cmds, const uint32_t *cmd_counts) {
+    if (!b || !gates || !chans || !chan_flags || !cmd_counts || nblocks == 0)
+        return fail(KST_EINVAL, "null argument or zero blocks");
+    HIPCHK(hipSetDevice(b->device));
+    std::vector<uint32_t> offs(nblocks);
+    size_t total = 0;
+    for (uint32_t i = 0; i < nblocks; ++i) {
+        offs[i] = (uint32_t)total;
+        total += cmd_counts[i];
+    }
+    if (total && !cmds) return fail(KST_EINVAL, "null spawn commands");
+    if (total > 0xFFFFFFFFull) return fail(KST_EINVAL, "too many staged spawn commands");
+    int rc = b->ensure_stage(nblocks, total);
+    if (rc) return rc;
+    hipStream_t s = b->stream;
+    HIPCHK(hipMemcpyAsync(b->g_gates.p, gates, (size_t)nblocks * b->gate_words * 4,
+                          hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(b->g_chans.p, chans, (size_t)nblocks * b->chan_words * 4,
+                          hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(b->g_chan_flags.p, chan_flags, nblocks * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(b->g_cmd_counts.p, cmd_counts, nblocks * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(b->g_cmd_offs.p, offs.data(), nblocks * 4, hipMemcpyHostToDevice, s));
+    if (total)
+        HIPCHK(hipMemcpyAsync(b->g_cmds.p, cmds, total * sizeof(kst_spawn_cmd),
+                              hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    b->staged = nblocks;
+    return KST_OK;
+}
+
+int kst_run_staged(kst_backend *b, int use_graph) {
+    if (!b) return fail(KST_EINVAL, "null argument");
+    if (!b->staged) return fail(KST_ESTATE, "no staged blocks");
+    HIPCHK(hipSetDevice(b->device));
+    hipStream_t s = b->stream;
+    uint32_t zero = 0;
+    HIPCHK(hipMemcpyAsync(&b->st.p->block, &zero, 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    // Capacity-sized grids: the live count is on the device only.
+    const uint32_t n = b->slots;
+    if (use_graph) {
+        if (!b->graph_exec) {
+            HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            b->launch_block(b->cfg.max_voices, n, true, true, true, false);
+            hipError_t e = hipStreamEndCapture(s, &b->graph);
+            if (e != hipSuccess) return fail(KST_EDEVICE, "graph capture: %s", hipGetErrorString(e));
+            HIPCHK(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
+        }
+        for (uint32_t i = 0; i < b->staged; ++i) HIPCHK(hipGraphLaunch(b->graph_exec, s));
+    } else {
+        for (uint32_t i = 0; i < b->staged; ++i)
+            b->launch_block(b->cfg.max_voices, n, true, true, true, false);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    // Bring the host mirror up to date from the last block's result row.
+    BlockResult r;
+    HIPCHK(hipMemcpy(&r, b->g_results.p + (b->staged - 1), sizeof r, hipMemcpyDeviceToHost));
+    b->live = (uint32_t)r.active_voices;
+    if (b->staged & 1) b->parity ^= 1;
+    b->stolen = r.stolen;
+    b->dropped = r.dropped;
+    std::memcpy(&b->peak, &r.peak_bits, 4);
+    return KST_OK;
+}
+
+int kst_staged_results(kst_backend *b, float *out, kst_block_stats *stats) {
+    if (!b) return fail(KST_EINVAL, "null argument");
+    if (!b->staged) return fail(KST_ESTATE, "no staged blocks");
+    HIPCHK(hipSetDevice(b->device));
+    const size_t two_f = 2 * (size_t)b->cfg.block_frames;
+    if (out)
+        HIPCHK(hipMemcpy(out, b->g_out.p, b->staged * two_f * 4, hipMemcpyDeviceToHost));
+    if (stats) {
+        std::vector<BlockResult> r(b->staged);
+        HIPCHK(hipMemcpy(r.data(), b->g_results.p, r.size() * sizeof(BlockResult),
+                         hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < b->staged; ++i) {
+            stats[i].active_voices = r[i].active_voices;
+            stats[i].stolen = r[i].stolen;
+            stats[i].dropped = r[i].dropped;
+            std::memcpy(&stats[i].peak, &r[i].peak_bits, 4);
+            stats[i].pad = 0;
+        }
+    }
+    return KST_OK;
+}
+
+int64_t kst_pool_export(kst_backend *b, uint32_t *soa, size_t stride) {
+    if (!b || !soa) return fail(KST_EINVAL, "null argument");
+    if (stride < b->live) return fail(KST_EINVAL, "stride %zu below the live count %u", stride, b->live);
+    if (hipSetDevice(b->device) != hipSuccess) return fail(KST_EDEVICE, "hipSetDevice");
+    for (uint32_t f = 0; f < VOICE_FIELDS; ++f) {
+        if (b->live == 0) break;
+        hipError_t e = hipMemcpy(soa + (size_t)f * stride, b->pools[b->parity].p + (size_t)f * b->slots,
+                                 (size_t)b->live * 4, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return fail(KST_EDEVICE, "pool export: %s", hipGetErrorString(e));
+    }
+    return (int64_t)b->live;
+}
+
+int kst_pool_import(kst_backend *b, const uint32_t *soa, size_t n, size_t stride) {
+    if (!b || (!soa && n)) return fail(KST_EINVAL, "null argument");
+    if (n > b->cfg.max_voices)
+        return fail(KST_EINVAL, "%zu voices exceed max_voices %u", n, b->cfg.max_voices);
+    if (stride < n) return fail(KST_EINVAL, "stride below n");
+    HIPCHK(hipSetDevice(b->device));
+    for (uint32_t f = 0; f < VOICE_FIELDS && n; ++f)
+        HIPCHK(hipMemcpy(b->pools[b->parity].p + (size_t)f * b->slots, soa + (size_t)f * stride,
+                         n * 4, hipMemcpyHostToDevice));
+    DevState s;
+    HIPCHK(hipMemcpy(&s, b->st.p, sizeof s, hipMemcpyDeviceToHost));
+    s.live = (uint32_t)n;
+    s.parity = b->parity;
+    HIPCHK(hipMemcpy(b->st.p, &s, sizeof s, hipMemcpyHostToDevice));
+    b->live = (uint32_t)n;
+    return KST_OK;
+}
+
+int kst_set_counters(kst_backend *b, uint64_t stolen, uint64_t dropped) {
+    if (!b) return fail(KST_EINVAL, "null argument");
+    HIPCHK(hipSetDevice(b->device));
+    DevState s;
+    HIPCHK(hipMemcpy(&s, b->st.p, sizeof s, hipMemcpyDeviceToHost));
+    s.stolen = stolen;
+    s.dropped = dropped;
+    HIPCHK(hipMemcpy(b->st.p, &s, sizeof s, hipMemcpyHostToDevice));
+    b->stolen = stolen;
+    b->dropped = dropped;
+    return KST_OK;
+}
+
+}  // extern "C"

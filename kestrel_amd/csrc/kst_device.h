@@ -1,0 +1,144 @@
+// kst_device.h -- layouts and kernel arguments shared by the gfx950 kernels and
+// the host orchestration of the Kestrel render backend.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/kestrel_hip.h"
+
+namespace kst {
+
+// ---- voice pool: 24 u32 fields, structure of arrays, field f of voice i at
+// f * capacity + i (shaders/common.wgsl:10-44) ----
+constexpr uint32_t VOICE_FIELDS = 24;
+enum : uint32_t {
+    F_PHASE_LO = 0, F_PHASE_HI = 1, F_STEP_LO = 2, F_STEP_HI = 3,
+    F_SMP_BASE = 4, F_SMP_LEN = 5, F_LOOP_START = 6, F_LOOP_END = 7,
+    F_FLAGS = 8, F_ENV_STAGE = 9, F_ENV_LEVEL = 10, F_GAIN_L = 11,
+    F_GAIN_R = 12, F_FILT_Z1 = 13, F_FILT_Z2 = 14, F_PARAMS = 15,
+    F_REGION = 16, F_GATE_SLOT = 17, F_ORDINAL = 18, F_START_REL = 19,
+    F_NOTE_LO = 20, F_NOTE_HI = 21, F_BORN_VARIANT = 22, F_STOP_REL = 23,
+};
+
+// Envelope stages (src/voice.rs:11-15), voice flags (src/bank.rs:12-14),
+// region-params flags (src/bank.rs:192).
+constexpr uint32_t ENV_ATTACK = 0, ENV_DECAY = 1, ENV_SUSTAIN = 2, ENV_RELEASE = 3,
+                   ENV_DEAD = 4;
+constexpr uint32_t VF_LOOP = 1, VF_LOOP_UNTIL_RELEASE = 2;
+constexpr uint32_t RP_FILTER = 1;
+
+// Gate and controller rows (src/voice.rs:18, 454-474).
+constexpr uint32_t GATE_SLOTS = 2048;
+constexpr uint32_t BEND_CHANNELS = 16, CHAN_FIELDS = 4;
+constexpr uint32_t CHAN_BEND = 0, CHAN_GAIN_L = 1, CHAN_GAIN_R = 2, CHAN_VARIANT = 3;
+constexpr uint32_t CHAN_ACTIVE_BEND = 1, CHAN_ACTIVE_GAIN = 2, CHAN_ACTIVE_VARIANT = 4;
+
+constexpr uint32_t STEAL_DROP_NEW = 1, STEAL_QUIETEST = 2;
+constexpr uint32_t ADMIT_EVEN = 1;
+
+// Radix select of the k-th smallest 64-bit steal key: eight MSB-first rounds
+// of one byte each (shaders/select.wgsl).
+constexpr uint32_t SEL_ROUNDS = 8;
+// Locality sort tiles: 256 threads x 8 items, processed in index order.
+constexpr uint32_t SORT_TPB = 256;
+constexpr uint32_t SORT_IPT = 8;
+constexpr uint32_t SORT_TILE = SORT_TPB * SORT_IPT;
+constexpr uint32_t MAX_SORT_PASSES = 32;
+
+// Wave64 render form: 16-frame sub-tiles, transposed through LDS with a row
+// stride of 68 words (68 = 4 mod 32 keeps the 128-bit column reads
+// bank-conflict free).
+constexpr uint32_t SUBTILE = 16;
+constexpr uint32_t TROW = 68;
+constexpr uint32_t TBUF_FLOATS = 2 * SUBTILE * TROW;  // 32 rows
+
+// Device-resident state of one backend, carried across blocks. The live count
+// lives here rather than in a kernel argument so a whole block of passes, or a
+// run of blocks, can be launched without the host in between
+// (src/gpu/mod.rs:1-7 made the same move for wgpu).
+struct DevState {
+    uint32_t live;    // compacted live count at the start of the block
+    uint32_t parity;  // which pool buffer holds the live pool
+    uint32_t block;   // index of the block within the staged inputs
+    uint32_t pad0;
+    unsigned long long stolen, dropped;  // cumulative
+    // Derived per block by k_begin (src/gpu/mod.rs:1015-1040).
+    uint32_t want, steal_k, spawn_count, live_total;
+    uint32_t cmd_off, chan_flags, live_new, peak_bits;
+    // Radix select: prefix and remaining rank entering each round.
+    uint32_t sel_prefix_hi[SEL_ROUNDS + 1];
+    uint32_t sel_prefix_lo[SEL_ROUNDS + 1];
+    uint32_t sel_k[SEL_ROUNDS + 1];
+    uint32_t thresh_hi, thresh_lo;
+};
+
+// Per-block result row, what stats() reads after each render.
+struct BlockResult {
+    unsigned long long active_voices, stolen, dropped;
+    uint32_t peak_bits, pad;
+};
+
+struct Args {
+    DevState *st;
+    uint32_t *pools[2];
+    uint32_t cap;  // allocated slots = SoA stride (Config::pool_slots)
+    uint32_t bank_len;
+    const int16_t *bank;
+    const kst_region_params *params;
+    uint32_t ppv;           // params per variant
+    uint32_t max_variants;
+    // Staged per-block inputs, indexed by st->block.
+    const uint32_t *gates;       // [nblocks][gate_words]
+    const uint32_t *chans;       // [nblocks][chan_words]
+    const uint32_t *chan_flags;  // [nblocks]
+    const kst_spawn_cmd *cmds;   // concatenated
+    const uint32_t *cmd_counts;  // [nblocks]
+    const uint32_t *cmd_offs;    // [nblocks]
+    uint32_t gate_words, chan_words;
+    float *out;               // [nblocks][2F]
+    BlockResult *results;     // [nblocks]
+    // Config.
+    uint32_t F, gate_frames, max_voices, max_steal, steal_span, fade;
+    uint32_t steal_rule, admit_rule, interp, exp_decay, exp_release;
+    float env_floor;
+    // Render / reduce.
+    float *partials;
+    uint32_t render_groups;  // rows of the partial buffer this block
+    uint32_t waves_per_group;
+    uint32_t plain_tile, plain_wg, kahan;
+    // Locality sort and compaction.
+    uint32_t *keys[2];
+    uint32_t *idx[2];
+    uint32_t *sort_hist;  // [passes][radix][tiles_max]
+    uint32_t *sort_tot;   // [passes][radix]
+    uint32_t sort_on, sort_passes, digit_bits, tiles_max;
+    uint32_t region_shift, stage_shift, phase_shift, phase_mask, dead_region;
+    uint32_t *sel_hist;  // [SEL_ROUNDS][256]
+};
+
+// One of a pair of buffers, chosen without indexing the kernel-argument array
+// (a dynamic index would force the pair out to scratch memory).
+template <typename T>
+__device__ __forceinline__ T *sel2(T *const (&pair)[2], uint32_t i) {
+    return i ? pair[1] : pair[0];
+}
+
+// ---- launchers (kst_pool.hip) ----
+void launch_begin(const Args &a, hipStream_t s);
+void launch_select(const Args &a, uint32_t grid, hipStream_t s);
+void launch_spawn(const Args &a, uint32_t grid, hipStream_t s);
+void launch_sort(const Args &a, uint32_t grid, hipStream_t s);
+void launch_end(const Args &a, hipStream_t s);
+
+// ---- launchers (kst_render.hip) ----
+// Returns false when the wave64 form does not cover this configuration.
+bool fast_render_supported(uint32_t F, uint32_t gate_frames, uint32_t interp);
+size_t fast_render_lds_bytes(uint32_t F, uint32_t waves_per_group, bool comp);
+void launch_render_fast(const Args &a, bool chan, bool comp, hipStream_t s);
+void launch_reduce_fast(const Args &a, bool comp, hipStream_t s);
+void launch_render_plain(const Args &a, bool chan, hipStream_t s);
+void launch_reduce_plain(const Args &a, hipStream_t s);
+
+}  // namespace kst

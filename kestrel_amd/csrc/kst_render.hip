@@ -1,0 +1,614 @@
+// kst_render.hip -- per-voice block render and the mixdown.
+//
+// Per-voice semantics follow src/cpu.rs:403-613 (= shaders/render.wgsl:146-421)
+// statement for statement, compiled with -ffp-contract=off so every f32
+// operation rounds where the reference rounds; the 32.32 phase is a native
+// uint64_t. Two forms:
+//
+//  * wave64 form (k_render_fast): one lane per voice, a persistent grid of
+//    waves walking the pool in 64-voice batches. Each 16-frame sub-tile the
+//    wave writes its 64 x 32 stereo contributions to LDS (row-major by output
+//    sample) and reads them back transposed, so a lane sums one output
+//    sample over 32 voices with plain adds and one cross-half exchange --
+//    about two VALU ops per voice-frame for the whole mixdown. Sums are
+//    accumulated per wave in LDS across batches (optionally as compensated
+//    hi/lo pairs), combined per workgroup, and written once as one partial
+//    row per workgroup.
+//  * plain form (k_render_plain + k_reduce_plain): the reference's structure
+//    -- reduce_tile frames per tile, an LDS staging array and a two-level tree
+//    per tile, partials[j * nwg + wg] accumulated across grid-stride batches,
+//    then one workgroup per output sample (shaders/render.wgsl:121-144,
+//    shaders/reduce.wgsl).
+//
+// No float atomics anywhere; every summation order is fixed, so two runs are
+// bitwise identical.
+#include "kst_device.h"
+
+namespace kst {
+namespace {
+
+constexpr float INV_32767 = 1.0f / 32767.0f;       // unpack2x16snorm scale, src/cpu.rs:193
+constexpr float FRAC_SCALE = 1.0f / 4294967296.0f;  // src/fixed.rs:73
+
+struct Voice {
+    uint64_t phase, step, base_step;
+    uint32_t base, len, ls, le, vflags, stage, gate_slot, ordinal;
+    uint32_t start_rel, stop_rel, params_base, variant, born_variant, region;
+    float level, z1, z2, gl, gr, bgl, bgr;
+    kst_region_params p;
+    bool live;
+};
+
+__device__ __forceinline__ kst_region_params load_params(const Args &a, uint32_t variant,
+                                                          uint32_t base) {
+    // src/cpu.rs:228-238: a variant outside the table reads the bank's own.
+    // Every variant slot is initialised with the bank's table, which covers a
+    // variant that was never uploaded.
+    // A params index past the table (the CPU reference would panic) reads
+    // entry 0 rather than out of bounds.
+    uint32_t v = variant < a.max_variants ? variant : 0;
+    uint32_t i = base < a.ppv ? base : 0;
+    return a.params[(size_t)v * a.ppv + i];
+}
+
+__device__ __forceinline__ void load_voice(const Args &a, const uint32_t *pool, uint32_t i,
+                                           bool live, Voice &v) {
+    const uint32_t c = a.cap;
+    v.live = live;
+    if (!live) {
+        v.stage = ENV_DEAD;
+        return;
+    }
+    v.phase = ((uint64_t)pool[F_PHASE_HI * c + i] << 32) | pool[F_PHASE_LO * c + i];
+    v.base_step = ((uint64_t)pool[F_STEP_HI * c + i] << 32) | pool[F_STEP_LO * c + i];
+    v.step = v.base_step;
+    v.base = pool[F_SMP_BASE * c + i];
+    v.len = pool[F_SMP_LEN * c + i];
+    v.ls = pool[F_LOOP_START * c + i];
+    v.le = pool[F_LOOP_END * c + i];
+    v.vflags = pool[F_FLAGS * c + i];
+    v.stage = pool[F_ENV_STAGE * c + i];
+    v.level = __uint_as_float(pool[F_ENV_LEVEL * c + i]);
+    v.bgl = __uint_as_float(pool[F_GAIN_L * c + i]);
+    v.bgr = __uint_as_float(pool[F_GAIN_R * c + i]);
+    v.gl = v.bgl;
+    v.gr = v.bgr;
+    v.z1 = __uint_as_float(pool[F_FILT_Z1 * c + i]);
+    v.z2 = __uint_as_float(pool[F_FILT_Z2 * c + i]);
+    v.params_base = pool[F_PARAMS * c + i];
+    v.region = pool[F_REGION * c + i];
+    v.gate_slot = pool[F_GATE_SLOT * c + i];
+    v.ordinal = pool[F_ORDINAL * c + i];
+    v.start_rel = pool[F_START_REL * c + i];
+    v.born_variant = pool[F_BORN_VARIANT * c + i];
+    v.stop_rel = pool[F_STOP_REL * c + i];
+    v.variant = v.born_variant > 0 ? v.born_variant - 1 : 0;
+    v.p = load_params(a, v.variant, v.params_base);
+}
+
+// Write back what the block changed, clear the per-block fields
+// (src/cpu.rs:605-612), and emit this voice's sort key (shaders/sort.wgsl:
+// 246-262) or its dead bit when sorting is off.
+__device__ __forceinline__ void store_voice(const Args &a, uint32_t *pool, uint32_t i,
+                                            const Voice &v) {
+    const uint32_t c = a.cap;
+    pool[F_PHASE_LO * c + i] = (uint32_t)v.phase;
+    pool[F_PHASE_HI * c + i] = (uint32_t)(v.phase >> 32);
+    pool[F_ENV_STAGE * c + i] = v.stage;
+    pool[F_ENV_LEVEL * c + i] = __float_as_uint(v.level);
+    pool[F_FILT_Z1 * c + i] = __float_as_uint(v.z1);
+    pool[F_FILT_Z2 * c + i] = __float_as_uint(v.z2);
+    pool[F_START_REL * c + i] = 0;
+    pool[F_BORN_VARIANT * c + i] = 0;
+    pool[F_STOP_REL * c + i] = 0;
+    uint32_t key;
+    if (a.sort_on) {
+        uint32_t region = a.dead_region;
+        if (v.stage != ENV_DEAD) region = v.region < a.dead_region - 1 ? v.region : a.dead_region - 1;
+        uint32_t ph = ((uint32_t)(v.phase >> 32) >> a.phase_shift) & a.phase_mask;
+        uint32_t st = v.stage < 7 ? v.stage : 7;
+        key = (region << a.region_shift) | (st << a.stage_shift) | ph;
+    } else {
+        key = v.stage == ENV_DEAD ? 1u : 0u;
+    }
+    a.keys[0][i] = key;
+}
+
+// 8.24 bend of a 32.32 step, truncating and saturating (src/fixed.rs:141-148).
+__device__ __forceinline__ uint64_t fixed_scale(uint64_t phase, uint32_t factor) {
+    uint64_t lo = phase * (uint64_t)factor;          // bits 0..63 of the product
+    uint64_t hi = __umul64hi(phase, (uint64_t)factor);  // bits 64..127
+    if (hi >> 24) return ~0ull;
+    return (hi << 40) | (lo >> 24);
+}
+
+// Note-off gate and controller row for one gate tile (src/cpu.rs:449-487).
+template <bool CHAN>
+__device__ __forceinline__ void gate_tile(const Args &a, const uint32_t *gates,
+                                          const uint32_t *chan, uint32_t chan_flags,
+                                          uint32_t gt, Voice &v) {
+    if (!v.live) return;
+    // The slot is masked to the row (the driver never builds one past 2047).
+    if (v.stage < ENV_RELEASE && gates[gt * GATE_SLOTS + (v.gate_slot & (GATE_SLOTS - 1))] >= v.ordinal) {
+        v.stage = ENV_RELEASE;
+        v.level = fminf(v.level, 1.0f);
+    }
+    if (CHAN && chan_flags) {
+        const uint32_t channel = (v.gate_slot >> 7) & 15;
+        const uint32_t ci = (gt * BEND_CHANNELS + channel) * CHAN_FIELDS;
+        if (chan_flags & CHAN_ACTIVE_BEND) v.step = fixed_scale(v.base_step, chan[ci + CHAN_BEND]);
+        if (chan_flags & CHAN_ACTIVE_GAIN) {
+            v.gl = v.bgl * __uint_as_float(chan[ci + CHAN_GAIN_L]);
+            v.gr = v.bgr * __uint_as_float(chan[ci + CHAN_GAIN_R]);
+        }
+        if (chan_flags & CHAN_ACTIVE_VARIANT) {
+            const uint32_t want = chan[ci + CHAN_VARIANT];
+            const uint32_t born_tile = v.start_rel / a.gate_frames;
+            const bool born_here = v.born_variant != 0 && gt <= born_tile;
+            if (want != v.variant && !born_here) {
+                v.variant = want;
+                v.p = load_params(a, v.variant, v.params_base);
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ float fetch(const Args &a, uint32_t base, uint32_t idx) {
+    // src/cpu.rs:190-194. u32 index arithmetic as in the reference; out of the
+    // pool reads zero.
+    const uint32_t i = base + idx;
+    const int v = i < a.bank_len ? (int)a.bank[i] : 0;
+    return fmaxf((float)v * INV_32767, -1.0f);
+}
+
+// src/cpu.rs:243-259
+__device__ __forceinline__ uint32_t advance_index(uint32_t idx, int32_t off, bool looping,
+                                                  uint32_t ls, uint32_t le, uint32_t len) {
+    const int64_t raw = (int64_t)idx + off;
+    if (looping) {
+        const int64_t l0 = ls, l1 = le;
+        if (raw >= l1) {
+            int64_t span = l1 - l0;
+            if (span < 1) span = 1;
+            return (uint32_t)(l0 + (raw - l0) % span);
+        }
+        if (raw < 0) return 0;
+        return (uint32_t)raw;
+    }
+    int64_t r = raw;
+    const int64_t hi = (int64_t)len - 1;
+    if (r < 0) r = 0;
+    if (r > hi) r = hi;
+    return (uint32_t)r;
+}
+
+template <int INTERP>
+__device__ __forceinline__ float interpolate(const Args &a, const Voice &v, uint32_t idx,
+                                             float frac, bool looping) {
+    if (INTERP == 0) return fetch(a, v.base, idx);
+    if (INTERP == 1) {
+        const uint32_t i1 = advance_index(idx, 1, looping, v.ls, v.le, v.len);
+        const float s0 = fetch(a, v.base, idx);
+        const float s1 = fetch(a, v.base, i1);
+        return s0 + (s1 - s0) * frac;
+    }
+    const uint32_t im1 = advance_index(idx, -1, looping, v.ls, v.le, v.len);
+    const uint32_t i1 = advance_index(idx, 1, looping, v.ls, v.le, v.len);
+    const uint32_t i2 = advance_index(idx, 2, looping, v.ls, v.le, v.len);
+    const float sm1 = fetch(a, v.base, im1);
+    const float s0 = fetch(a, v.base, idx);
+    const float s1 = fetch(a, v.base, i1);
+    const float s2 = fetch(a, v.base, i2);
+    // src/cpu.rs:297-302, Horner form.
+    const float ca = -0.5f * sm1 + 1.5f * s0 - 1.5f * s1 + 0.5f * s2;
+    const float cb = sm1 - 2.5f * s0 + 2.0f * s1 - 0.5f * s2;
+    const float cc = -0.5f * sm1 + 0.5f * s1;
+    return ((ca * frac + cb) * frac + cc) * frac + s0;
+}
+
+// One frame of one voice, src/cpu.rs:491-601. Returns the (y*gain_l,
+// y*gain_r) contribution, or zeros when the voice is silent this frame.
+template <int INTERP>
+__device__ __forceinline__ void frame(const Args &a, uint32_t f, Voice &v, float &ol,
+                                      float &orr) {
+    ol = 0.0f;
+    orr = 0.0f;
+    if (!v.live || v.stage == ENV_DEAD || f < v.start_rel) return;
+    const bool loop_en = (v.vflags & VF_LOOP) != 0;
+    const bool lur = (v.vflags & VF_LOOP_UNTIL_RELEASE) != 0;
+    const bool looping = loop_en && !(lur && v.stage >= ENV_RELEASE);
+    const uint32_t idx = (uint32_t)(v.phase >> 32);
+    if (!looping && idx >= v.len) {
+        v.stage = ENV_DEAD;
+        return;
+    }
+    // Envelope first, so an instant attack is at full level on frame 0.
+    switch (v.stage) {
+        case ENV_ATTACK:
+            v.level = v.level + v.p.attack_rate;
+            if (v.level >= v.p.attack_end) {
+                v.stage = ENV_DECAY;
+                v.level = 1.0f;
+            }
+            break;
+        case ENV_DECAY:
+            v.level = a.exp_decay ? v.level * v.p.decay_coef : v.level - v.p.decay_coef;
+            if (v.level <= v.p.decay_target) {
+                if (v.p.sustain <= a.env_floor) {
+                    v.stage = ENV_DEAD;
+                    v.level = 0.0f;
+                } else {
+                    v.stage = ENV_SUSTAIN;
+                    v.level = v.p.sustain;
+                }
+            }
+            break;
+        case ENV_RELEASE:
+            v.level = a.exp_release ? v.level * v.p.release_coef : v.level - v.p.release_coef;
+            if (v.level <= a.env_floor) {
+                v.stage = ENV_DEAD;
+                v.level = 0.0f;
+            }
+            break;
+        default:
+            break;
+    }
+    const float frac = (float)(uint32_t)v.phase * FRAC_SCALE;
+    const float s = interpolate<INTERP>(a, v, idx, frac, looping);
+    const float g = fminf(v.level, 1.0f);
+    const float x = s * g;
+    float y = x;
+    if (v.p.flags & RP_FILTER) {
+        // Transposed direct form II, b2 == b0.
+        y = v.p.b0 * x + v.z1;
+        v.z1 = v.p.b1 * x - v.p.a1 * y + v.z2;
+        v.z2 = v.p.b0 * x - v.p.a2 * y;
+    }
+    // A stolen voice fades over steal_fade_frames from its own stop frame,
+    // after the filter.
+    if (v.stop_rel != 0 && f + 1 >= v.stop_rel) {
+        const uint32_t d = f + 1 - v.stop_rel;
+        if (d >= a.fade) {
+            v.stage = ENV_DEAD;
+            y = 0.0f;
+        } else {
+            y = y * (1.0f - (float)d / (float)a.fade);
+        }
+    }
+    ol = y * v.gl;
+    orr = y * v.gr;
+    v.phase += v.step;
+    if (looping) {
+        const uint32_t hi = (uint32_t)(v.phase >> 32);
+        if (hi >= v.le) {
+            uint32_t span = v.le - v.ls;
+            if (span < 1) span = 1;
+            const uint32_t wrapped = v.ls + (hi - v.ls) % span;
+            v.phase = ((uint64_t)wrapped << 32) | (v.phase & 0xFFFFFFFFull);
+        }
+    }
+}
+
+// Error-free transformation of a + b (Knuth's two-sum).
+__device__ __forceinline__ void two_sum(float a, float b, float &s, float &e) {
+    s = a + b;
+    const float bp = s - a;
+    e = (a - (s - bp)) + (b - bp);
+}
+
+// ---------------------------------------------------------------------------
+// wave64 form
+// ---------------------------------------------------------------------------
+
+// Per-wave LDS: the 32 x 68 transposition buffer, then 2F accumulated output
+// samples (and 2F compensation terms).
+__host__ __device__ constexpr uint32_t wave_lds_floats(uint32_t F, bool comp) {
+    return TBUF_FLOATS + 2 * F * (comp ? 2 : 1);
+}
+
+template <int INTERP, bool CHAN, bool COMP>
+__global__ void __launch_bounds__(1024) k_render_fast(Args a) {
+    extern __shared__ float lds[];
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t wpg = blockDim.x >> 6;
+    const uint32_t F = a.F;
+    const uint32_t two_f = 2 * F;
+    float *tb = lds + wave * wave_lds_floats(F, COMP);
+    float *acc = tb + TBUF_FLOATS;
+    float *accl = acc + two_f;
+
+    DevState *st = a.st;
+    const uint32_t live_total = st->live_total;
+    const uint32_t b = st->block;
+    const uint32_t chan_flags = CHAN ? st->chan_flags : 0u;
+    uint32_t *pool = sel2(a.pools, st->parity);
+    const uint32_t *gates = a.gates + (size_t)b * a.gate_words;
+    const uint32_t *chan = a.chans + (size_t)b * a.chan_words;
+
+    for (uint32_t j = lane; j < two_f; j += 64) {
+        acc[j] = 0.0f;
+        if (COMP) accl[j] = 0.0f;
+    }
+
+    const uint32_t half = lane >> 5;  // which 32 voices this lane sums over
+    const uint32_t o = lane & 31;     // which output of the sub-tile it owns
+    const uint32_t nw = gridDim.x * wpg;
+    for (uint32_t batch = blockIdx.x * wpg + wave; batch * 64 < live_total; batch += nw) {
+        const uint32_t vi = batch * 64 + lane;
+        Voice v;
+        load_voice(a, pool, vi, vi < live_total, v);
+        for (uint32_t s = 0; s < F / SUBTILE; ++s) {
+            const uint32_t f0 = s * SUBTILE;
+            if (f0 % a.gate_frames == 0) gate_tile<CHAN>(a, gates, chan, chan_flags, f0 / a.gate_frames, v);
+#pragma unroll
+            for (uint32_t i = 0; i < SUBTILE; ++i) {
+                float ol, orr;
+                frame<INTERP>(a, f0 + i, v, ol, orr);
+                tb[(2 * i) * TROW + lane] = ol;
+                tb[(2 * i + 1) * TROW + lane] = orr;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // Transposed read: output o over the 32 voices of this half.
+            const float4 *row = reinterpret_cast<const float4 *>(tb + o * TROW + 32 * half);
+            float4 q[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) q[k] = row[k];
+            float t[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) t[k] = (q[k].x + q[k].y) + (q[k].z + q[k].w);
+            const float sum = ((t[0] + t[1]) + (t[2] + t[3])) + ((t[4] + t[5]) + (t[6] + t[7]));
+            const float other = __shfl_xor(sum, 32);
+            // Same value in both halves: (h0 + h1) == (h1 + h0).
+            const float total = half ? other + sum : sum + other;
+            // Lanes of half h own output 32*s + o; accumulate where s%2 == h so
+            // each output sample is added by exactly one lane.
+            if ((s & 1) == half) {
+                const uint32_t j = s * 32 + o;
+                if (COMP) {
+                    float hi, e;
+                    two_sum(acc[j], total, hi, e);
+                    acc[j] = hi;
+                    accl[j] = accl[j] + e;
+                } else {
+                    acc[j] = acc[j] + total;
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        if (v.live) store_voice(a, pool, vi, v);
+    }
+
+    // Workgroup combine in wave order, then one partial row per workgroup.
+    __syncthreads();
+    float *prow = a.partials + (size_t)blockIdx.x * two_f * (COMP ? 2 : 1);
+    for (uint32_t j = threadIdx.x; j < two_f; j += blockDim.x) {
+        float hi = lds[TBUF_FLOATS + j];
+        float lo = COMP ? lds[TBUF_FLOATS + two_f + j] : 0.0f;
+        for (uint32_t w = 1; w < wpg; ++w) {
+            const float *wl = lds + w * wave_lds_floats(F, COMP) + TBUF_FLOATS;
+            if (COMP) {
+                float s, e;
+                two_sum(hi, wl[j], s, e);
+                hi = s;
+                lo = lo + (e + wl[two_f + j]);
+            } else {
+                hi = hi + wl[j];
+            }
+        }
+        prow[j] = hi;
+        if (COMP) prow[two_f + j] = lo;
+    }
+}
+
+// Column sums of the partial rows: out[j] = sum over workgroup rows, in row
+// order, 64 columns per workgroup and 4 row phases per column.
+template <bool COMP>
+__global__ void __launch_bounds__(256) k_reduce_fast(Args a) {
+    DevState *st = a.st;
+    const uint32_t two_f = 2 * a.F;
+    const uint32_t rows = a.render_groups;
+    const uint32_t col = blockIdx.x * 64 + (threadIdx.x & 63);
+    const uint32_t ph = threadIdx.x >> 6;
+    __shared__ float sh[2][4][64];
+    float hi = 0.0f, lo = 0.0f;
+    if (col < two_f) {
+        for (uint32_t r = ph; r < rows; r += 4) {
+            const float *row = a.partials + (size_t)r * two_f * (COMP ? 2 : 1);
+            if (COMP) {
+                float s, e;
+                two_sum(hi, row[col], s, e);
+                hi = s;
+                lo = lo + (e + row[two_f + col]);
+            } else {
+                hi = hi + row[col];
+            }
+        }
+    }
+    sh[0][ph][threadIdx.x & 63] = hi;
+    sh[1][ph][threadIdx.x & 63] = lo;
+    __syncthreads();
+    if (ph == 0 && col < two_f) {
+        float h = sh[0][0][threadIdx.x], l = sh[1][0][threadIdx.x];
+        for (uint32_t k = 1; k < 4; ++k) {
+            if (COMP) {
+                float s, e;
+                two_sum(h, sh[0][k][threadIdx.x], s, e);
+                h = s;
+                l = l + (e + sh[1][k][threadIdx.x]);
+            } else {
+                h = h + sh[0][k][threadIdx.x];
+            }
+        }
+        const float v = COMP ? h + l : h;
+        a.out[(size_t)st->block * two_f + col] = v;
+        const float m = fabsf(v);
+        if (m == m) atomicMax(&st->peak_bits, __float_as_uint(m));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// plain form: the reference's render + reduce structure
+// ---------------------------------------------------------------------------
+
+template <int INTERP, bool CHAN>
+__global__ void __launch_bounds__(1024) k_render_plain(Args a) {
+    extern __shared__ float lds[];
+    const uint32_t WG = blockDim.x;
+    const uint32_t TILE = a.plain_tile;
+    const uint32_t M = TILE * 2;
+    const uint32_t PER_LANE = WG / M;
+    float *sh = lds;             // [M][WG]
+    float *sh2 = lds + WG * M;   // [WG]
+    const uint32_t tid = threadIdx.x, wg = blockIdx.x, nwg = gridDim.x;
+    DevState *st = a.st;
+    const uint32_t live = st->live_total;
+    const uint32_t b = st->block;
+    const uint32_t chan_flags = CHAN ? st->chan_flags : 0u;
+    uint32_t *pool = sel2(a.pools, st->parity);
+    const uint32_t *gates = a.gates + (size_t)b * a.gate_words;
+    const uint32_t *chan = a.chans + (size_t)b * a.chan_words;
+    const uint32_t samples = a.F * 2;
+
+    for (uint32_t j = tid; j < samples; j += WG) a.partials[(size_t)j * nwg + wg] = 0.0f;
+    __syncthreads();
+
+    for (uint32_t batch = wg; batch * WG < live; batch += nwg) {
+        const uint32_t vi = batch * WG + tid;
+        Voice v;
+        load_voice(a, pool, vi, vi < live, v);
+        const uint32_t tiles = a.F / TILE;
+        const uint32_t tiles_per_gate = a.gate_frames / TILE;
+        for (uint32_t tile = 0; tile < tiles; ++tile) {
+            if (tile % tiles_per_gate == 0)
+                gate_tile<CHAN>(a, gates, chan, chan_flags, tile / tiles_per_gate, v);
+            const uint32_t f0 = tile * TILE;
+            for (uint32_t i = 0; i < TILE; ++i) {
+                float ol, orr;
+                frame<INTERP>(a, f0 + i, v, ol, orr);
+                sh[(i * 2) * WG + tid] = ol;
+                sh[(i * 2 + 1) * WG + tid] = orr;
+            }
+            __syncthreads();
+            // shaders/render.wgsl:125-144
+            const uint32_t lane = tid / PER_LANE, chunk = tid % PER_LANE;
+            float acc = 0.0f;
+            for (uint32_t k = 0; k < M; ++k) acc = acc + sh[lane * WG + chunk + k * PER_LANE];
+            sh2[lane * PER_LANE + chunk] = acc;
+            __syncthreads();
+            if (tid < M) {
+                float total = 0.0f;
+                for (uint32_t k = 0; k < PER_LANE; ++k) total = total + sh2[tid * PER_LANE + k];
+                const size_t j = (size_t)(f0 * 2 + tid) * nwg + wg;
+                a.partials[j] = a.partials[j] + total;
+            }
+            __syncthreads();
+        }
+        if (v.live) store_voice(a, pool, vi, v);
+    }
+}
+
+// shaders/reduce.wgsl:110-150
+__global__ void __launch_bounds__(1024) k_reduce_plain(Args a) {
+    extern __shared__ float red[];
+    DevState *st = a.st;
+    const uint32_t j = blockIdx.x, tid = threadIdx.x, WG = blockDim.x;
+    const uint32_t nwg = a.render_groups;
+    const size_t base = (size_t)j * nwg;
+    float acc = 0.0f;
+    if (a.kahan) {
+        float comp = 0.0f;
+        for (uint32_t w = tid; w < nwg; w += WG) {
+            const float y = a.partials[base + w] - comp;
+            const float t = acc + y;
+            comp = (t - acc) - y;
+            acc = t;
+        }
+    } else {
+        for (uint32_t w = tid; w < nwg; w += WG) acc = acc + a.partials[base + w];
+    }
+    red[tid] = acc;
+    __syncthreads();
+    for (uint32_t s = WG / 2; s > 0; s >>= 1) {
+        if (tid < s) red[tid] = red[tid] + red[tid + s];
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const float v = red[0];
+        a.out[(size_t)st->block * 2 * a.F + j] = v;
+        const float m = fabsf(v);
+        if (m == m) atomicMax(&st->peak_bits, __float_as_uint(m));
+    }
+}
+
+template <int INTERP, bool CHAN, bool COMP>
+void launch_fast_t(const Args &a, hipStream_t s) {
+    const uint32_t threads = a.waves_per_group * 64;
+    const size_t lds = (size_t)a.waves_per_group * wave_lds_floats(a.F, COMP) * sizeof(float);
+    if (lds > 65536) {
+        // gfx950 gives a workgroup up to 160 KiB of LDS; past 64 KiB the
+        // kernel has to ask for it.
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_render_fast<INTERP, CHAN, COMP>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    }
+    k_render_fast<INTERP, CHAN, COMP><<<a.render_groups, threads, lds, s>>>(a);
+}
+
+template <int INTERP, bool CHAN>
+void launch_plain_t(const Args &a, hipStream_t s) {
+    const uint32_t WG = a.plain_wg;
+    const size_t lds = (size_t)WG * (a.plain_tile * 2 + 1) * sizeof(float);
+    k_render_plain<INTERP, CHAN><<<a.render_groups, WG, lds, s>>>(a);
+}
+
+}  // namespace
+
+bool fast_render_supported(uint32_t F, uint32_t gate_frames, uint32_t interp) {
+    return F % 32 == 0 && gate_frames % SUBTILE == 0 && interp <= 2;
+}
+
+size_t fast_render_lds_bytes(uint32_t F, uint32_t waves_per_group, bool comp) {
+    return (size_t)waves_per_group * wave_lds_floats(F, comp) * sizeof(float);
+}
+
+void launch_render_fast(const Args &a, bool chan, bool comp, hipStream_t s) {
+#define KST_FAST(I)                                                        \
+    if (a.interp == I) {                                                   \
+        if (chan) {                                                        \
+            comp ? launch_fast_t<I, true, true>(a, s) : launch_fast_t<I, true, false>(a, s); \
+        } else {                                                           \
+            comp ? launch_fast_t<I, false, true>(a, s) : launch_fast_t<I, false, false>(a, s); \
+        }                                                                  \
+        return;                                                            \
+    }
+    KST_FAST(0)
+    KST_FAST(1)
+    KST_FAST(2)
+#undef KST_FAST
+}
+
+void launch_reduce_fast(const Args &a, bool comp, hipStream_t s) {
+    const uint32_t cols = 2 * a.F;
+    const uint32_t grid = (cols + 63) / 64;
+    if (comp)
+        k_reduce_fast<true><<<grid, 256, 0, s>>>(a);
+    else
+        k_reduce_fast<false><<<grid, 256, 0, s>>>(a);
+}
+
+void launch_render_plain(const Args &a, bool chan, hipStream_t s) {
+    if (a.interp == 0) chan ? launch_plain_t<0, true>(a, s) : launch_plain_t<0, false>(a, s);
+    else if (a.interp == 1) chan ? launch_plain_t<1, true>(a, s) : launch_plain_t<1, false>(a, s);
+    else chan ? launch_plain_t<2, true>(a, s) : launch_plain_t<2, false>(a, s);
+}
+
+void launch_reduce_plain(const Args &a, hipStream_t s) {
+    const uint32_t WG = a.plain_wg;
+    k_reduce_plain<<<2 * a.F, WG, WG * sizeof(float), s>>>(a);
+}
+
+}  // namespace kst

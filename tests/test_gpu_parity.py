@@ -1,0 +1,210 @@
+"""Parity of the gfx950 backend with the CPU oracle on identical inputs.
+
+Bar (stated per check below):
+* integer state -- phase accumulators, envelope stages, voice order, steal
+  victims (stop frames), compaction order -- bit-exact, compared slot by slot
+  (the oracle runs in sorted mode when the device sorts);
+* float voice state (level, filter memory) bit-exact too: each voice's f32
+  arithmetic is the reference's, operation for operation;
+* the mix: the device sums in f32 in a fixed tree order, the oracle in f64, so
+  the mix nulls to NULL_DB or better relative to the block's reference peak.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from kestrel_amd import workload as W
+from kestrel_amd.config import Config
+
+pytestmark = pytest.mark.gpu
+
+NULL_DB = -100.0
+
+
+def make_pair(cfg, bank, sorted_mode=None):
+    from kestrel_amd.backend import HipSynth
+    from oracle import oracle
+    gpu = HipSynth(cfg, bank.pool, bank.params, bank.n_regions, bank.max_sample_len)
+    cpu = oracle.Oracle(cfg, bank.pool, bank.params)
+    if sorted_mode if sorted_mode is not None else cfg.sort_voices:
+        cpu.set_sorted(bank.n_regions, bank.max_sample_len, True)
+    return gpu, cpu
+
+
+def null_db(a, ref):
+    d = float(np.max(np.abs(a.astype(np.float64) - ref.astype(np.float64))))
+    peak = float(np.max(np.abs(ref)))
+    if d == 0.0:
+        return -math.inf
+    return 20 * math.log10(d / max(peak, 1e-30))
+
+
+def step_both(gpu, cpu, blk):
+    for be in (gpu, cpu):
+        be.set_gates(blk.gates)
+        be.set_channels(blk.chans, *blk.flags)
+        be.spawn(blk.cmds)
+    sg, sc = gpu.stats(), cpu.stats()
+    assert (sg["active_voices"], sg["stolen"], sg["dropped"]) == \
+        (sc["active_voices"], sc["stolen"], sc["dropped"])
+    og = gpu.render()
+    oc = cpu.render()
+    return og, oc
+
+
+def compare_pools(gpu, cpu, where):
+    pg, pc = gpu.pool_export(), cpu.pool_export()
+    assert pg.shape == pc.shape, f"{where}: live {pg.shape[1]} vs {pc.shape[1]}"
+    bad = np.flatnonzero((pg != pc).any(axis=1))
+    assert bad.size == 0, f"{where}: fields {bad.tolist()} differ"
+
+
+def run_stream(cfg, bank, builder, blocks, check_pool_every=1, min_live=1):
+    gpu, cpu = make_pair(cfg, bank)
+    worst = -math.inf
+    max_live = 0
+    for b in range(blocks):
+        blk = builder.next_block()
+        og, oc = step_both(gpu, cpu, blk)
+        worst = max(worst, null_db(og, oc))
+        sg, sc = gpu.stats(), cpu.stats()
+        assert (sg["active_voices"], sg["stolen"], sg["dropped"]) == \
+            (sc["active_voices"], sc["stolen"], sc["dropped"]), b
+        max_live = max(max_live, sg["active_voices"])
+        if check_pool_every and b % check_pool_every == 0:
+            compare_pools(gpu, cpu, f"block {b}")
+    assert max_live >= min_live
+    assert worst <= NULL_DB, worst
+    gpu.close()
+    return worst
+
+
+@pytest.mark.parametrize("form,sort", [(1, True), (1, False), (0, True), (0, False)])
+def test_c1_stream_bit_exact_state_and_null(form, sort):
+    cfg, bank, notes = W.workload_c1()
+    cfg = cfg.replace(form=form, sort_voices=sort)
+    print(f"c1 form={form} sort={sort}", flush=True)
+    run_stream(cfg, bank, W.BlockBuilder(cfg, bank, notes), 120, min_live=100)
+
+
+@pytest.mark.parametrize("interp", [0, 2])
+def test_c1_other_interpolations(interp):
+    cfg, bank, notes = W.workload_c1()
+    cfg = cfg.replace(interpolation=interp)
+    run_stream(cfg, bank, W.BlockBuilder(cfg, bank, notes), 60, min_live=100)
+
+
+@pytest.mark.parametrize("rule,admit", [(2, 0), (0, 0), (1, 1), (2, 1)])
+def test_saturated_small_pool_steals_and_drops(rule, admit):
+    """Pool of 512 under ~3000 note-ons a second: steal/drop every block."""
+    cfg = Config(block_frames=256, max_voices=512, steal_rule=rule, admit_rule=admit,
+                 max_steal_percent=10)
+    bank = W.make_bank(5, instruments=8, zones=4, key_lo=21, key_hi=108)
+    notes = W.poisson_notes(6, 12_000, 4.0, key_lo=21, key_hi=108, mean_dur_s=1.0)
+    print(f"saturated rule={rule} admit={admit}", flush=True)
+    run_stream(cfg, bank, W.BlockBuilder(cfg, bank, notes), 150, min_live=500)
+
+
+def test_bends_drive_the_controller_path():
+    cfg, bank, notes = W.workload_c1()
+    cfg = cfg.replace(interpolation=2)
+    bb = W.BlockBuilder(cfg, bank, notes, bends=W.random_walk_bends(9, 2.0, 0.05))
+    run_stream(cfg, bank, bb, 60, min_live=100)
+
+
+def test_compensated_form_matches_and_nulls_deeper():
+    cfg, bank, notes = W.workload_c1()
+    w_plain = run_stream(cfg, bank, W.BlockBuilder(cfg, bank, notes), 80, min_live=100)
+    cfgc = cfg.replace(compensated=True)
+    w_comp = run_stream(cfgc, bank, W.BlockBuilder(cfgc, bank, notes), 80, min_live=100)
+    assert w_comp <= w_plain + 1.0
+
+
+def test_gain_and_variant_rows():
+    """Channel gains and a params variant switch mid-stream, as CC7/CC74 would."""
+    cfg, bank, notes = W.workload_c1()
+    gpu, cpu = make_pair(cfg, bank)
+    variant = bank.params.copy()
+    variant["b0"] *= np.float32(0.5)
+    variant["b1"] *= np.float32(0.5)
+    for be in (gpu, cpu):
+        be.set_params_variant(1, variant)
+    bb = W.BlockBuilder(cfg, bank, notes)
+    worst = -math.inf
+    for b in range(60):
+        blk = bb.next_block()
+        ch = blk.chans.reshape(cfg.tiles(), 16, 4).copy()
+        if b >= 20:
+            ch[:, 3, 1] = np.float32(0.5).view(np.uint32)
+            ch[:, 3, 2] = np.float32(0.25).view(np.uint32)
+        if b >= 30:
+            ch[cfg.tiles() // 2:, 5, 3] = 1
+        blk.chans = ch.ravel()
+        blk.flags = (False, b >= 20, b >= 30)
+        og, oc = step_both(gpu, cpu, blk)
+        worst = max(worst, null_db(og, oc))
+        compare_pools(gpu, cpu, f"block {b}")
+    assert worst <= NULL_DB
+
+
+def test_staged_run_equals_per_block_calls():
+    cfg, bank, notes = W.workload_c1()
+    from kestrel_amd.backend import HipSynth
+    blocks = [blk for blk in (W.BlockBuilder(cfg, bank, notes).next_block() for _ in range(40))]
+    a = HipSynth(cfg, bank.pool, bank.params, bank.n_regions, bank.max_sample_len)
+    outs = []
+    for blk in blocks:
+        a.set_gates(blk.gates)
+        a.set_channels(blk.chans, *blk.flags)
+        a.spawn(blk.cmds)
+        outs.append(a.render().copy())
+    pool_a = a.pool_export()
+    for use_graph in (False, True):
+        b = HipSynth(cfg, bank.pool, bank.params, bank.n_regions, bank.max_sample_len)
+        b.stage_blocks(np.concatenate([x.gates for x in blocks]),
+                       np.concatenate([x.chans for x in blocks]),
+                       np.array([x.chan_flag_word for x in blocks], np.uint32),
+                       np.concatenate([x.cmds for x in blocks]),
+                       np.array([len(x.cmds) for x in blocks], np.uint32))
+        b.run_staged(use_graph)
+        out, stats = b.staged_results()
+        assert np.array_equal(out, np.stack(outs)), use_graph
+        assert np.array_equal(b.pool_export(), pool_a)
+        assert stats[-1]["active_voices"] == a.stats()["active_voices"]
+        b.close()
+
+
+def test_reruns_are_byte_identical():
+    cfg, bank, notes = W.workload_c1()
+    from kestrel_amd.backend import HipSynth
+    res = []
+    for _ in range(2):
+        g = HipSynth(cfg, bank.pool, bank.params, bank.n_regions, bank.max_sample_len)
+        bb = W.BlockBuilder(cfg, bank, notes)
+        outs = []
+        for _ in range(40):
+            blk = bb.next_block()
+            g.set_gates(blk.gates)
+            g.set_channels(blk.chans, *blk.flags)
+            g.spawn(blk.cmds)
+            outs.append(g.render().copy())
+        res.append(np.concatenate(outs))
+        g.close()
+    assert res[0].tobytes() == res[1].tobytes()
+
+
+def test_validation_errors_name_the_flag():
+    from kestrel_amd.backend import HipSynth
+    from kestrel_amd.abi import BackendError
+    cfg, bank, _ = W.workload_c1()
+    g = HipSynth(cfg, bank.pool, bank.params, bank.n_regions, bank.max_sample_len)
+    with pytest.raises(BackendError, match="gate rows"):
+        g.set_gates(np.zeros(5, np.uint32))
+    with pytest.raises(BackendError, match="past the configured maximum"):
+        g.set_params_variant(cfg.max_param_variants, bank.params)
+    with pytest.raises(BackendError, match="entries, expected"):
+        g.set_params_variant(1, bank.params[:3])
+    with pytest.raises(BackendError, match="output block"):
+        g.render(np.zeros(7, np.float32))
+    g.close()
