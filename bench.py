@@ -1,0 +1,237 @@
+#!/usr/bin/env python3
+"""Headline benchmark: voice-samples/s and x-realtime at ~1M voices.
+
+Workload: the plain method at saturation -- 48 kHz stereo, 512-frame blocks, a
+1,048,576-voice pool with steal-percent 10, linear interpolation, a 16 x 4-zone
+bank of 32768-frame int16 samples, and dense bursts of note-ons (calibrated so
+the pool stays saturated and stealing fires every block).
+
+A "step" is one block of the whole hot path on the device: steal select and
+marking, spawn, render, mixdown, locality sort and compaction. `--warmup` blocks
+fill the pool; then the inputs of `--steps` blocks are staged in HBM and run
+back to back (the timed region). Reported beside it:
+
+* plain: the same blocks through the reference-structured render and reduce,
+  one synchronous Backend::render call per block, as the reference runs;
+* cpu_baseline: the CPU oracle (a restatement of src/cpu.rs, one thread)
+  rendering the first timed block from the same pool snapshot, whose output
+  is also the parity check (null in dB re the block peak);
+* roofline: the render kernel's algorithmic bytes per launch over its mean
+  duration (HIP events on the backend's own stream), against 8 TB/s.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "voice-samples/s & x-realtime at 1M voices; vs plain GPU path at <=-100dB null"
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def null_db(a, ref):
+    d = float(np.max(np.abs(np.asarray(a, np.float64) - np.asarray(ref, np.float64))))
+    peak = float(np.max(np.abs(ref)))
+    return -math.inf if d == 0.0 else 20.0 * math.log10(d / max(peak, 1e-30))
+
+
+def stage(backend, blocks):
+    backend.stage_blocks(np.concatenate([b.gates for b in blocks]),
+                         np.concatenate([b.chans for b in blocks]),
+                         np.array([b.chan_flag_word for b in blocks], np.uint32),
+                         np.concatenate([b.cmds for b in blocks]),
+                         np.array([len(b.cmds) for b in blocks], np.uint32))
+
+
+def render_voices(cfg, live0, blocks, stats):
+    """Voices entering render per block: live + spawned (src/gpu/mod.rs:1015-1040)."""
+    out = []
+    live = live0
+    for b, st in zip(blocks, stats):
+        out.append(live + cfg.admit_take(live, len(b.cmds)))
+        live = st["active_voices"]
+    return np.array(out, np.float64)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=320)
+    ap.add_argument("--compensated", type=int, default=1)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--graph", action="store_true",
+                    help="replay the block chain from a hipGraph (no per-pass events)")
+    ap.add_argument("--seed", type=int, default=0x4B530002)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the render path is single-GPU, see DESIGN.md for how it would shard)")
+        sys.exit(2)
+
+    from kestrel_amd import workload as W
+    from kestrel_amd.backend import HipSynth
+
+    K, WU = args.steps, args.warmup
+    cfg = W.config_c2(compensated=bool(args.compensated), profile=not args.graph)
+    seconds = (WU + K + 4) * cfg.block_frames / cfg.sample_rate
+    t = time.time()
+    _, bank, notes = W.workload_c2(args.seed, seconds=seconds, cfg=cfg,
+                                   base_rate_hz=500_000.0)
+    builder = W.BlockBuilder(cfg, bank, notes)
+    log(f"workload: {len(notes)} notes over {seconds:.2f} s, bank {bank.n_regions} regions, "
+        f"built in {time.time() - t:.1f} s")
+
+    gpu = HipSynth(cfg, bank.pool, bank.params, bank.n_regions, bank.max_sample_len)
+    # ---- warmup: fill the pool (untimed) ----
+    t = time.time()
+    done = 0
+    while done < WU:
+        n = min(64, WU - done)
+        chunk = [builder.next_block() for _ in range(n)]
+        stage(gpu, chunk)
+        gpu.run_staged(use_graph=False)
+        done += n
+    st0 = gpu.stats()
+    log(f"warmup: {WU} blocks in {time.time() - t:.1f} s, live {st0['active_voices']}, "
+        f"stolen {st0['stolen']}, dropped {st0['dropped']}")
+    snapshot = gpu.pool_export()
+    live0 = snapshot.shape[1]
+
+    timed = [builder.next_block() for _ in range(K)]
+    stage(gpu, timed)
+    # ---- timed region: K blocks, inputs resident in HBM ----
+    t0 = time.perf_counter()
+    gpu.run_staged(use_graph=args.graph)
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    outs, stats = gpu.staged_results()
+    timings = dict(gpu.timings())
+    V = render_voices(cfg, live0, timed, stats)
+    F = cfg.block_frames
+    voice_samples = float(V.sum()) * F
+    value = voice_samples / wall
+    audio_s = K * F / cfg.sample_rate
+    x_rt = audio_s / wall
+    steals = [b["stolen"] - a["stolen"] for a, b in zip([st0] + stats[:-1], stats)]
+    log(f"timed: {K} blocks in {wall * 1e3:.1f} ms ({wall / K * 1e3:.3f} ms/block), "
+        f"mean voices {V.mean():.0f}, {value:.3e} voice-samples/s, {x_rt:.1f}x realtime, "
+        f"blocks with steals {sum(1 for s in steals if s > 0)}/{K}")
+    log(f"per-pass ms/block: {timings}")
+    pool_final = gpu.pool_export()
+
+    # ---- roofline of the render kernel ----
+    step = (snapshot[3].astype(np.float64) * 2 ** 32 + snapshot[2]) / 2 ** 32
+    mean_step = float(step.mean()) if step.size else 1.0
+    taps = 2
+    spawned = np.array([min(len(b.cmds), cfg.max_voices) for b in timed], np.float64)
+    bytes_block = V * (192.0 + 2.0 * (taps - 1) + 2.0 * F * mean_step) + 76.0 * spawned + 8.0 * F
+    per_unit = float(bytes_block.sum() / voice_samples)
+    render_ms = timings.get("render")
+    roofline = None
+    if render_ms:
+        achieved = float(bytes_block.mean()) / (render_ms * 1e-3) / 1e9
+        traffic = None
+        tf = os.path.join(ROOT, "profiles", "render_traffic.json")
+        if os.path.exists(tf):
+            with open(tf) as f:
+                traffic = json.load(f).get("bytes_per_launch")
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                    "traffic": traffic, "kernel": "k_render_fast",
+                    "bytes_per_voice_sample": round(per_unit, 4),
+                    "render_ms_per_block": round(render_ms, 4)}
+    log(f"roofline: {roofline}")
+
+    # ---- CPU baseline + full-size parity on the first timed block ----
+    cpu_baseline = None
+    parity = {}
+    if not args.no_cpu:
+        from oracle import oracle
+        o = oracle.Oracle(cfg, bank.pool, bank.params)
+        o.set_sorted(bank.n_regions, bank.max_sample_len, True)
+        o.pool_import(snapshot)
+        o.set_counters(st0["stolen"], st0["dropped"])
+        blk = timed[0]
+        o.set_gates(blk.gates)
+        o.set_channels(blk.chans, *blk.flags)
+        o.spawn(blk.cmds)
+        tc = time.perf_counter()
+        oc = o.render()
+        tcpu = time.perf_counter() - tc
+        cpu_vs = V[0] * F / tcpu
+        cpu_baseline = {"value": round(cpu_vs, 1), "unit": "voice-samples/s", "cores": 1,
+                        "kind": "port",
+                        "sample": f"1 block of {F} frames at {int(V[0])} voices "
+                                  f"({tcpu:.1f} s, single-threaded restatement of src/cpu.rs)",
+                        "x_realtime": round(F / cfg.sample_rate / tcpu, 4)}
+        parity["null_db_vs_cpu"] = round(null_db(outs[0], oc), 1)
+        parity["stats_match_cpu"] = o.stats()["active_voices"] == stats[0]["active_voices"]
+        log(f"cpu: {cpu_baseline}, null vs cpu {parity['null_db_vs_cpu']} dB")
+        oracle_pool = o.pool_export()
+        del o
+
+    # ---- plain form: reference-structured kernels, one synchronous call per block ----
+    plain = None
+    if not args.no_plain:
+        pcfg = cfg.replace(form=0, compensated=False, profile=False)
+        p = HipSynth(pcfg, bank.pool, bank.params, bank.n_regions, bank.max_sample_len)
+        p.pool_import(snapshot)
+        p.set_counters(st0["stolen"], st0["dropped"])
+        pouts = np.empty_like(outs)
+        tp = time.perf_counter()
+        for i, blk in enumerate(timed):
+            p.set_gates(blk.gates)
+            p.set_channels(blk.chans, *blk.flags)
+            p.spawn(blk.cmds)
+            p.render(pouts[i])
+            if i == 0 and not args.no_cpu:
+                parity["pool_bit_exact_vs_cpu"] = bool(np.array_equal(p.pool_export(),
+                                                                      oracle_pool))
+        tp = time.perf_counter() - tp
+        plain_vs = voice_samples / tp
+        parity["plain_vs_chosen_null_db"] = round(null_db(pouts, outs), 1)
+        parity["pools_equal_plain_vs_chosen"] = bool(np.array_equal(p.pool_export(), pool_final))
+        plain = {"value": round(plain_vs, 1), "unit": "voice-samples/s",
+                 "ms_per_step": round(tp / K * 1e3, 4), "x_realtime": round(audio_s / tp, 2),
+                 "form": "reference-structured render+reduce, host sync per block"}
+        log(f"plain: {plain}, parity {parity}")
+        p.close()
+
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "voice-samples/s",
+        "n_gpus": 1, "steps": K, "warmup": WU, "ms_per_step": round(wall / K * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "plain method at saturation: 48 kHz stereo, 512-frame blocks, "
+                               "1,048,576-voice pool, steal-percent 10, linear interpolation, "
+                               "16x4-zone bank of 32768-frame int16 samples, burst note-ons",
+                   "block_frames": F, "max_voices": cfg.max_voices,
+                   "mean_voices": round(float(V.mean()), 1),
+                   "form": gpu.name(), "graph": bool(args.graph)},
+        "x_realtime": round(x_rt, 2),
+        "roofline": roofline,
+        "cpu_baseline": cpu_baseline,
+        "plain": plain,
+        "vs_plain": round(value / plain["value"], 3) if plain else None,
+        "parity": parity,
+    }
+    gpu.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -90,6 +90,7 @@ typedef struct kst_config {
     uint32_t groups_per_cu;         /* wave64 form: persistent render groups per CU */
     uint32_t sort_digit_bits;       /* radix width of the locality sort, 1..8 */
     int32_t device;                 /* HIP device ordinal */
+    uint32_t profile;               /* per-pass timings (src/config.rs:337-338) */
 } kst_config;
 
 typedef struct kst_backend kst_backend;
@@ -136,7 +137,8 @@ int kst_stats(const kst_backend *b, kst_block_stats *out);
 const char *kst_name(const kst_backend *b);
 
 /* Backend::timings -- src/backend.rs:53-57. Writes up to `cap` entries and
- * returns how many there are. Empty until a block has rendered. */
+ * returns how many there are. Empty until a block has rendered with `profile`
+ * on; after a staged run without a graph, each pass's mean over its blocks. */
 size_t kst_timings(const kst_backend *b, kst_timing *out, size_t cap);
 
 /* The message of the last failure on this thread (anyhow's error text). */

@@ -55,6 +55,7 @@ class Config:
     groups_per_cu: int = 3
     sort_digit_bits: int = 8
     device: int = 0
+    profile: bool = False
 
     def validate(self) -> None:
         """src/config.rs:412-525, the render-path subset."""
@@ -122,7 +123,8 @@ class KstConfig(C.Structure):
         "decay_curve", "release_curve")] + [("env_floor", C.c_float)] + [
         (n, C.c_uint32) for n in ("max_param_variants", "kahan_reduce", "form",
                                   "compensated", "waves_per_group", "groups_per_cu",
-                                  "sort_digit_bits")] + [("device", C.c_int32)]
+                                  "sort_digit_bits")] + [("device", C.c_int32),
+                                                         ("profile", C.c_uint32)]
 
     @classmethod
     def from_config(cls, cfg: Config) -> "KstConfig":

@@ -175,6 +175,7 @@ struct kst_backend {
     DevBuf<uint32_t> pools[2];
     DevBuf<int16_t> bank;
     uint32_t bank_len = 0;
+    uint32_t bank_clamp = 1;
     DevBuf<kst_region_params> params;
     DevBuf<float> partials;
     DevBuf<uint32_t> keys[2], idx[2], sort_hist, sort_tot, sel_hist;
@@ -204,6 +205,7 @@ struct kst_backend {
 
     bool profile = false;
     hipEvent_t ev[6] = {};
+    std::vector<hipEvent_t> stage_ev;
     std::vector<kst_timing> timings;
 
     hipGraph_t graph = nullptr;
@@ -214,6 +216,7 @@ struct kst_backend {
         if (graph) (void)hipGraphDestroy(graph);
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
+        for (auto &e : stage_ev) (void)hipEventDestroy(e);
         for (auto &p : pools) p.release();
         bank.release();
         params.release();
@@ -292,6 +295,7 @@ struct kst_backend {
         a.pools[1] = pools[1].p;
         a.cap = slots;
         a.bank_len = bank_len;
+        a.clamp = bank_clamp;
         a.bank = bank.p;
         a.params = params.p;
         a.ppv = ppv;
@@ -347,7 +351,8 @@ struct kst_backend {
     // the true counts from the device state, so capacity-sized grids (staged
     // runs) are as correct as exact ones (per-block calls).
     void launch_block(uint32_t n_live, uint32_t n_total, bool may_steal, bool spawn, bool chan,
-                      bool timed) {
+                      hipEvent_t *ev) {
+        const bool timed = ev != nullptr;
         const uint32_t rg = render_groups_for(n_total);
         Args a = args(rg);
         hipStream_t s = stream;
@@ -372,13 +377,27 @@ struct kst_backend {
         if (timed) (void)hipEventRecord(ev[5], s);
     }
 
-    void read_timings() {
+    // Per-pass milliseconds, averaged over `nblocks` event sets of 6.
+    void read_timings(const hipEvent_t *evs, uint32_t nblocks) {
         timings.clear();
         for (int i = 0; i < 5; ++i) {
-            float ms = 0.0f;
-            if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) != hipSuccess) ms = 0.0f;
-            timings.push_back({PASS_NAMES[i], (double)ms});
+            double sum = 0.0;
+            for (uint32_t k = 0; k < nblocks; ++k) {
+                float ms = 0.0f;
+                if (hipEventElapsedTime(&ms, evs[6 * k + i], evs[6 * k + i + 1]) == hipSuccess)
+                    sum += ms;
+            }
+            timings.push_back({PASS_NAMES[i], sum / nblocks});
         }
+    }
+
+    int ensure_stage_events(uint32_t nblocks) {
+        while (stage_ev.size() < 6ull * nblocks) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            stage_ev.push_back(e);
+        }
+        return KST_OK;
     }
 };
 
@@ -411,6 +430,7 @@ void kst_default_config(kst_config *c) {
     c->groups_per_cu = 3;
     c->sort_digit_bits = 8;
     c->device = 0;
+    c->profile = 0;
 }
 
 const char *kst_last_error(void) { return g_last_error.c_str(); }
@@ -475,7 +495,13 @@ int kst_create(const kst_config *cfg, const int16_t *pool, size_t pool_len,
     // every neighbour read of the last frame stays inside the buffer.
     std::vector<int16_t> padded(pool, pool + pool_len);
     padded.resize(pool_len + 8 + (pool_len & 1), 0);
-    b->bank_len = (uint32_t)std::min<size_t>(pool_len, 0xFFFFFFFFu);
+    if (pool_len >= (1ull << 31))
+        return cleanup(fail(KST_EINVAL, "sample pool of %zu frames exceeds the 2^31-frame limit "
+                                        "of 32-bit byte offsets", pool_len));
+    b->bank_len = (uint32_t)pool_len;
+    // -32768 is the one value the -1 clamp of src/cpu.rs:193 changes; a bank
+    // without it lets the taps skip the clamp.
+    b->bank_clamp = std::find(pool, pool + pool_len, (int16_t)-32768) != pool + pool_len;
     if (b->bank.alloc(padded.size()) != hipSuccess)
         return cleanup(fail(KST_EDEVICE, "allocating the %zu-sample pool", padded.size()));
     if (hipMemcpy(b->bank.p, padded.data(), padded.size() * 2, hipMemcpyHostToDevice) != hipSuccess)
@@ -528,7 +554,7 @@ int kst_create(const kst_config *cfg, const int16_t *pool, size_t pool_len,
     for (auto &x : b->ev)
         if (hipEventCreate(&x) != hipSuccess)
             return cleanup(fail(KST_EDEVICE, "creating timing events"));
-    b->profile = getenv("KST_PROFILE") != nullptr;
+    b->profile = c.profile != 0 || getenv("KST_PROFILE") != nullptr;
     *out = b;
     return KST_OK;
 }
@@ -633,7 +659,8 @@ int kst_render(kst_backend *b, float *out, size_t n) {
         need = std::min<uint64_t>(std::min<uint64_t>(live + want - cap, live), b->max_steal);
     const uint64_t take = std::min<uint64_t>(want, cap - (live - need));
     const uint32_t total = (uint32_t)(live + take);
-    b->launch_block((uint32_t)live, total, need > 0, take > 0, b->h_chan_flags != 0, b->profile);
+    b->launch_block((uint32_t)live, total, need > 0, take > 0, b->h_chan_flags != 0,
+                    b->profile ? b->ev : nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(b->pin_out, b->g_out.p, two_f * 4, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(b->pin_result, b->g_results.p, sizeof(BlockResult),
@@ -651,7 +678,7 @@ int kst_render(kst_backend *b, float *out, size_t n) {
                                 "(%llu/%llu)",
                     (unsigned long long)r.stolen, (unsigned long long)r.dropped,
                     (unsigned long long)b->stolen, (unsigned long long)b->dropped);
-    if (b->profile) b->read_timings();
+    if (b->profile) b->read_timings(b->ev, 1);
     b->pending.clear();
     return KST_OK;
 }
@@ -734,18 +761,24 @@ int kst_run_staged(kst_backend *b, int use_graph) {
     if (use_graph) {
         if (!b->graph_exec) {
             HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            b->launch_block(b->cfg.max_voices, n, true, true, true, false);
+            b->launch_block(b->cfg.max_voices, n, true, true, true, nullptr);
             hipError_t e = hipStreamEndCapture(s, &b->graph);
             if (e != hipSuccess) return fail(KST_EDEVICE, "graph capture: %s", hipGetErrorString(e));
             HIPCHK(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
         }
         for (uint32_t i = 0; i < b->staged; ++i) HIPCHK(hipGraphLaunch(b->graph_exec, s));
     } else {
+        if (b->profile) {
+            int rc = b->ensure_stage_events(b->staged);
+            if (rc) return rc;
+        }
         for (uint32_t i = 0; i < b->staged; ++i)
-            b->launch_block(b->cfg.max_voices, n, true, true, true, false);
+            b->launch_block(b->cfg.max_voices, n, true, true, true,
+                            b->profile ? &b->stage_ev[6 * i] : nullptr);
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipStreamSynchronize(s));
+    if (b->profile && !use_graph) b->read_timings(b->stage_ev.data(), b->staged);
     // Bring the host mirror up to date from the last block's result row.
     BlockResult r;
     HIPCHK(hipMemcpy(&r, b->g_results.p + (b->staged - 1), sizeof r, hipMemcpyDeviceToHost));

@@ -85,6 +85,7 @@ struct Args {
     uint32_t *pools[2];
     uint32_t cap;  // allocated slots = SoA stride (Config::pool_slots)
     uint32_t bank_len;
+    uint32_t clamp;  // the bank holds -32768, so taps need the -1 clamp
     const int16_t *bank;
     const kst_region_params *params;
     uint32_t ppv;           // params per variant

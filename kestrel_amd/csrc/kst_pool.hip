@@ -215,16 +215,19 @@ __device__ __forceinline__ bool key_alive(const Args &a, uint32_t key) {
     return a.sort_on ? (key >> a.region_shift) < a.dead_region : key == 0;
 }
 
-__global__ void __launch_bounds__(SORT_TPB) k_sort_hist(Args a) {
+// Per-tile digit counts of pass p over that pass's input order (the previous
+// pass's output), plus the digit totals. Pass 0 also counts the survivors.
+__global__ void __launch_bounds__(SORT_TPB) k_sort_hist(Args a, uint32_t p) {
     DevState *st = a.st;
     const uint32_t n = st->live_total;
     const uint32_t tiles = (n + SORT_TILE - 1) / SORT_TILE;
     const uint32_t radix = 1u << a.digit_bits, mask = radix - 1;
-    extern __shared__ uint32_t sh[];  // [passes][radix]
+    const uint32_t shift = p * a.digit_bits;
+    __shared__ uint32_t sh[256];
     __shared__ uint32_t s_alive;
-    const uint32_t *keys = a.keys[0];
+    const uint32_t *keys = sel2(a.keys, p & 1);
     for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
-        for (uint32_t i = threadIdx.x; i < a.sort_passes * radix; i += SORT_TPB) sh[i] = 0;
+        for (uint32_t i = threadIdx.x; i < radix; i += SORT_TPB) sh[i] = 0;
         if (threadIdx.x == 0) s_alive = 0;
         __syncthreads();
         uint32_t alive = 0;
@@ -232,16 +235,15 @@ __global__ void __launch_bounds__(SORT_TPB) k_sort_hist(Args a) {
             uint32_t item = t * SORT_TILE + k * SORT_TPB + threadIdx.x;
             if (item >= n) break;
             uint32_t key = keys[item];
-            alive += key_alive(a, key) ? 1u : 0u;
-            for (uint32_t p = 0; p < a.sort_passes; ++p)
-                atomicAdd(&sh[p * radix + ((key >> (p * a.digit_bits)) & mask)], 1u);
+            if (p == 0) alive += key_alive(a, key) ? 1u : 0u;
+            atomicAdd(&sh[(key >> shift) & mask], 1u);
         }
         if (alive) atomicAdd(&s_alive, alive);
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < a.sort_passes * radix; i += SORT_TPB) {
-            uint32_t c = sh[i];
-            a.sort_hist[(size_t)i * a.tiles_max + t] = c;
-            if (c) atomicAdd(&a.sort_tot[i], c);
+        for (uint32_t d = threadIdx.x; d < radix; d += SORT_TPB) {
+            uint32_t c = sh[d];
+            a.sort_hist[(size_t)(p * radix + d) * a.tiles_max + t] = c;
+            if (c) atomicAdd(&a.sort_tot[p * radix + d], c);
         }
         if (threadIdx.x == 0 && s_alive) atomicAdd(&st->live_new, s_alive);
         __syncthreads();
@@ -251,13 +253,12 @@ __global__ void __launch_bounds__(SORT_TPB) k_sort_hist(Args a) {
 // One workgroup per (pass, digit): the digit's base is the sum of the smaller
 // digits' totals; each tile's offset adds the same digit's counts in the tiles
 // before it.
-__global__ void __launch_bounds__(256) k_sort_scan(Args a) {
+__global__ void __launch_bounds__(256) k_sort_scan(Args a, uint32_t p) {
     DevState *st = a.st;
     const uint32_t n = st->live_total;
     const uint32_t tiles = (n + SORT_TILE - 1) / SORT_TILE;
     const uint32_t radix = 1u << a.digit_bits;
-    const uint32_t p = blockIdx.x / radix, d = blockIdx.x % radix;
-    if (p >= a.sort_passes) return;
+    const uint32_t d = blockIdx.x;
     __shared__ uint32_t red[256];
     uint32_t s = 0;
     for (uint32_t i = threadIdx.x; i < d; i += 256) s += a.sort_tot[p * radix + i];
@@ -391,10 +392,11 @@ void launch_spawn(const Args &a, uint32_t grid, hipStream_t s) {
 
 void launch_sort(const Args &a, uint32_t grid, hipStream_t s) {
     const uint32_t radix = 1u << a.digit_bits;
-    size_t lds = (size_t)a.sort_passes * radix * sizeof(uint32_t);
-    k_sort_hist<<<grid, SORT_TPB, lds, s>>>(a);
-    k_sort_scan<<<a.sort_passes * radix, 256, 0, s>>>(a);
-    for (uint32_t p = 0; p < a.sort_passes; ++p) k_sort_scatter<<<grid, SORT_TPB, 0, s>>>(a, p);
+    for (uint32_t p = 0; p < a.sort_passes; ++p) {
+        k_sort_hist<<<grid, SORT_TPB, 0, s>>>(a, p);
+        k_sort_scan<<<radix, 256, 0, s>>>(a, p);
+        k_sort_scatter<<<grid, SORT_TPB, 0, s>>>(a, p);
+    }
     k_gather<<<grid, 256, 0, s>>>(a);
 }
 

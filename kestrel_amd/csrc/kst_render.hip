@@ -56,7 +56,16 @@ __device__ __forceinline__ void load_voice(const Args &a, const uint32_t *pool, 
     const uint32_t c = a.cap;
     v.live = live;
     if (!live) {
+        // A neutral lane: silent, and every address it could form is in bounds.
+        v.phase = v.step = v.base_step = 0;
+        v.base = 0;
+        v.len = v.ls = v.le = 1;
+        v.vflags = 0;
         v.stage = ENV_DEAD;
+        v.gate_slot = v.ordinal = v.start_rel = v.stop_rel = 0;
+        v.params_base = v.variant = v.born_variant = v.region = 0;
+        v.level = v.z1 = v.z2 = v.gl = v.gr = v.bgl = v.bgr = 0.0f;
+        v.p = kst_region_params{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         return;
     }
     v.phase = ((uint64_t)pool[F_PHASE_HI * c + i] << 32) | pool[F_PHASE_LO * c + i];
@@ -289,6 +298,183 @@ __device__ __forceinline__ void frame(const Args &a, uint32_t f, Voice &v, float
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// wave64 fast path: a 16-frame sub-tile in which no lane can reach a loop end,
+// a sample end, its start offset or a steal fade. Then the per-frame work is
+// straight-line: the envelope is one fused multiply-add per frame -- exact,
+// because in every stage one of its two operations is exact (x*1 or +(-0)) --
+// with a compare that sends the rare stage change to a uniform fix-up branch,
+// and the sample taps are plain neighbours.
+// ---------------------------------------------------------------------------
+
+constexpr float F_INF = __builtin_huge_valf();
+
+// level' = fmaf(level, m, c), stage change when level' >= thi or level' <= tlo
+// (src/cpu.rs:512-549).
+__device__ __forceinline__ void env_coefs(const Args &a, const Voice &v, float &m, float &c,
+                                          float &thi, float &tlo) {
+    m = 1.0f;
+    c = -0.0f;
+    thi = F_INF;
+    tlo = -F_INF;
+    switch (v.stage) {
+        case ENV_ATTACK:
+            c = v.p.attack_rate;
+            thi = v.p.attack_end;
+            break;
+        case ENV_DECAY:
+            if (a.exp_decay) m = v.p.decay_coef; else c = -v.p.decay_coef;
+            tlo = v.p.decay_target;
+            break;
+        case ENV_RELEASE:
+            if (a.exp_release) m = v.p.release_coef; else c = -v.p.release_coef;
+            tlo = a.env_floor;
+            break;
+        default:
+            break;
+    }
+}
+
+// The stage change of src/cpu.rs:512-549 for a lane whose compare fired.
+__device__ __forceinline__ void env_transition(const Args &a, Voice &v) {
+    if (v.stage == ENV_ATTACK) {
+        v.stage = ENV_DECAY;
+        v.level = 1.0f;
+    } else if (v.stage == ENV_DECAY) {
+        if (v.p.sustain <= a.env_floor) {
+            v.stage = ENV_DEAD;
+            v.level = 0.0f;
+        } else {
+            v.stage = ENV_SUSTAIN;
+            v.level = v.p.sustain;
+        }
+    } else if (v.stage == ENV_RELEASE) {
+        v.stage = ENV_DEAD;
+        v.level = 0.0f;
+    }
+}
+
+// Can this lane take the straight-line path for frames [f0, f0 + SUBTILE)?
+// Silent lanes (no voice, or DEAD) always can: they are neutralised.
+template <int INTERP>
+__device__ __forceinline__ bool fast_ok(const Args &a, const Voice &v, uint32_t f0) {
+    if (!v.live || v.stage == ENV_DEAD) return true;
+    if (v.start_rel > f0) return false;
+    if (v.stop_rel != 0 && v.stop_rel <= f0 + SUBTILE) return false;
+    const uint32_t hi = (uint32_t)(v.phase >> 32);
+    // No 64-bit overflow in phase + SUBTILE * step.
+    if ((v.step >> 32) >= (1u << 26) || hi >= 0x80000000u) return false;
+    const uint32_t hi_end = (uint32_t)((v.phase + SUBTILE * v.step) >> 32);
+    const bool loop_en = (v.vflags & VF_LOOP) != 0;
+    const bool lur = (v.vflags & VF_LOOP_UNTIL_RELEASE) != 0;
+    const bool looping = loop_en && !(lur && v.stage >= ENV_RELEASE);
+    const uint32_t limit = looping ? v.le : v.len;
+    const uint32_t reach = INTERP == 2 ? 2u : (INTERP == 1 ? 1u : 0u);
+    if ((uint64_t)hi_end + reach >= limit) return false;
+    if (INTERP == 2 && hi < 1) return false;
+    // Every tap stays inside the bank.
+    if ((uint64_t)v.base + hi_end + reach >= a.bank_len) return false;
+    return true;
+}
+
+template <bool CLAMP>
+__device__ __forceinline__ float tap(const char *bank, uint32_t byte_off) {
+    const int x = *reinterpret_cast<const int16_t *>(bank + byte_off);
+    const float f = (float)x * INV_32767;
+    return CLAMP ? fmaxf(f, -1.0f) : f;
+}
+
+// FILT: 0 every live lane unfiltered, 1 every live lane filtered, 2 mixed.
+template <int INTERP, bool CLAMP, int FILT>
+__device__ __forceinline__ void fast_subtile(const Args &a, Voice &v, float *tb, uint32_t lane,
+                                             bool env_static) {
+    if (!v.live || v.stage == ENV_DEAD) {
+        v.gl = v.gr = 0.0f;
+        v.step = 0;
+        v.phase = (uint64_t)(INTERP == 2 ? 1u : 0u) << 32;
+        v.base = 0;
+    }
+    float m, c, thi, tlo;
+    env_coefs(a, v, m, c, thi, tlo);
+    const char *bank = reinterpret_cast<const char *>(a.bank);
+    const uint32_t base2 = v.base * 2u;
+    const bool filt = FILT == 1 || (FILT == 2 && (v.p.flags & RP_FILTER));
+    float g = fminf(v.level, 1.0f);
+    bool kill = false;
+#pragma unroll
+    for (uint32_t i = 0; i < SUBTILE; ++i) {
+        if (!env_static) {
+            if (kill) {
+                // A lane that died on the previous frame is silent from here.
+                if (v.stage == ENV_DEAD) v.gl = v.gr = 0.0f;
+                kill = false;
+            }
+            v.level = fmaf(v.level, m, c);
+            const bool tr = (v.level >= thi) | (v.level <= tlo);
+            if (__builtin_expect(__any(tr), 0)) {
+                if (tr) {
+                    env_transition(a, v);
+                    env_coefs(a, v, m, c, thi, tlo);
+                }
+                kill = __any(tr && v.stage == ENV_DEAD);
+            }
+            g = fminf(v.level, 1.0f);
+        }
+        const uint32_t idx = (uint32_t)(v.phase >> 32);
+        const uint32_t off = base2 + idx * 2u;
+        const float flo = (float)(uint32_t)v.phase;
+        float s;
+        if (INTERP == 0) {
+            s = tap<CLAMP>(bank, off);
+        } else if (INTERP == 1) {
+            const float s0 = tap<CLAMP>(bank, off);
+            const float s1 = tap<CLAMP>(bank, off + 2);
+            // s0 + (s1 - s0) * (flo * 2^-32): the power-of-two scale commutes
+            // with rounding, so one fused step gives the same bits.
+            s = fmaf((s1 - s0) * flo, FRAC_SCALE, s0);
+        } else {
+            const float sm1 = tap<CLAMP>(bank, off - 2);
+            const float s0 = tap<CLAMP>(bank, off);
+            const float s1 = tap<CLAMP>(bank, off + 2);
+            const float s2 = tap<CLAMP>(bank, off + 4);
+            const float frac = flo * FRAC_SCALE;
+            const float ca = -0.5f * sm1 + 1.5f * s0 - 1.5f * s1 + 0.5f * s2;
+            const float cb = sm1 - 2.5f * s0 + 2.0f * s1 - 0.5f * s2;
+            const float cc = -0.5f * sm1 + 0.5f * s1;
+            s = ((ca * frac + cb) * frac + cc) * frac + s0;
+        }
+        const float x = s * g;
+        float y = x;
+        if (FILT == 1 || (FILT == 2 && filt)) {
+            const float b0x = v.p.b0 * x;
+            y = b0x + v.z1;
+            v.z1 = v.p.b1 * x - v.p.a1 * y + v.z2;
+            v.z2 = b0x - v.p.a2 * y;
+        }
+        tb[(2 * i) * TROW + lane] = y * v.gl;
+        tb[(2 * i + 1) * TROW + lane] = y * v.gr;
+        v.phase += v.step;
+    }
+    if (kill && v.stage == ENV_DEAD) v.gl = v.gr = 0.0f;
+}
+
+template <int INTERP, bool CLAMP>
+__device__ __forceinline__ void fast_subtile_dispatch(const Args &a, Voice &v, float *tb,
+                                                      uint32_t lane) {
+    const bool sounding = v.live && v.stage != ENV_DEAD;
+    const bool env_static = !__any(sounding && v.stage != ENV_SUSTAIN);
+    const bool f = (v.p.flags & RP_FILTER) != 0;
+    if (!__any(sounding && !f)) {
+        env_static ? fast_subtile<INTERP, CLAMP, 1>(a, v, tb, lane, true)
+                   : fast_subtile<INTERP, CLAMP, 1>(a, v, tb, lane, false);
+    } else if (!__any(sounding && f)) {
+        fast_subtile<INTERP, CLAMP, 0>(a, v, tb, lane, env_static);
+    } else {
+        fast_subtile<INTERP, CLAMP, 2>(a, v, tb, lane, env_static);
+    }
+}
+
 // Error-free transformation of a + b (Knuth's two-sum).
 __device__ __forceinline__ void two_sum(float a, float b, float &s, float &e) {
     s = a + b;
@@ -341,12 +527,18 @@ __global__ void __launch_bounds__(1024) k_render_fast(Args a) {
         for (uint32_t s = 0; s < F / SUBTILE; ++s) {
             const uint32_t f0 = s * SUBTILE;
             if (f0 % a.gate_frames == 0) gate_tile<CHAN>(a, gates, chan, chan_flags, f0 / a.gate_frames, v);
-#pragma unroll
-            for (uint32_t i = 0; i < SUBTILE; ++i) {
-                float ol, orr;
-                frame<INTERP>(a, f0 + i, v, ol, orr);
-                tb[(2 * i) * TROW + lane] = ol;
-                tb[(2 * i + 1) * TROW + lane] = orr;
+            if (!__any(!fast_ok<INTERP>(a, v, f0))) {
+                if (a.clamp)
+                    fast_subtile_dispatch<INTERP, true>(a, v, tb, lane);
+                else
+                    fast_subtile_dispatch<INTERP, false>(a, v, tb, lane);
+            } else {
+                for (uint32_t i = 0; i < SUBTILE; ++i) {
+                    float ol, orr;
+                    frame<INTERP>(a, f0 + i, v, ol, orr);
+                    tb[(2 * i) * TROW + lane] = ol;
+                    tb[(2 * i + 1) * TROW + lane] = orr;
+                }
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();

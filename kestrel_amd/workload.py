@@ -151,7 +151,7 @@ class Bank:
         span = (self.key_hi[0] - self.key_lo[0] + 1)
         zone = np.clip((key.astype(np.int64) - int(self.key_lo[0])) // int(span), 0,
                        self.zones - 1)
-        return instrument.astype(np.int64) * self.zones + zone
+        return (instrument.astype(np.int64) % self.instruments) * self.zones + zone
 
 
 def sine_noise_sample(rng: np.random.Generator, n: int, freq: float, rate: int,

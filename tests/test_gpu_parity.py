@@ -208,3 +208,48 @@ def test_validation_errors_name_the_flag():
     with pytest.raises(BackendError, match="output block"):
         g.render(np.zeros(7, np.float32))
     g.close()
+
+
+def test_multi_tile_sort_and_steal_at_40k_voices():
+    """A pool large enough for many sort tiles and select workgroups, saturated
+    so every block steals; the oracle runs in sorted mode."""
+    cfg = W.config_c2(max_voices=40_000)
+    bank = W.make_bank(11, instruments=16, zones=4, key_lo=0, key_hi=127)
+    notes = W.burst_notes(12, 0.8, base_rate_hz=150_000.0, burst_notes_n=20_000,
+                          burst_every_s=0.25)
+    print("40k voices", flush=True)
+    run_stream(cfg, bank, W.BlockBuilder(cfg, bank, notes), 60, check_pool_every=5,
+               min_live=39_000)
+
+
+def test_one_block_at_full_size_matches_the_oracle():
+    """The bench workload at ~1M voices: fill the pool on the device, then one
+    block from the same snapshot on both sides -- pool bit-exact, mix nulled."""
+    from kestrel_amd.backend import HipSynth
+    from oracle import oracle
+    cfg = W.config_c2()
+    _, bank, notes = W.workload_c2(seconds=3.6, cfg=cfg, base_rate_hz=500_000.0)
+    bb = W.BlockBuilder(cfg, bank, notes)
+    g = HipSynth(cfg, bank.pool, bank.params, bank.n_regions, bank.max_sample_len)
+    for _ in range(5):
+        chunk = [bb.next_block() for _ in range(64)]
+        g.stage_blocks(np.concatenate([b.gates for b in chunk]),
+                       np.concatenate([b.chans for b in chunk]),
+                       np.array([b.chan_flag_word for b in chunk], np.uint32),
+                       np.concatenate([b.cmds for b in chunk]),
+                       np.array([len(b.cmds) for b in chunk], np.uint32))
+        g.run_staged(False)
+    st = g.stats()
+    print("live after warmup", st, flush=True)
+    assert st["active_voices"] > 1_000_000
+    snap = g.pool_export()
+    o = oracle.Oracle(cfg, bank.pool, bank.params)
+    o.set_sorted(bank.n_regions, bank.max_sample_len, True)
+    o.pool_import(snap)
+    o.set_counters(st["stolen"], st["dropped"])
+    blk = bb.next_block()
+    og, oc = step_both(g, o, blk)
+    assert g.stats()["stolen"] > st["stolen"]   # this block steals
+    compare_pools(g, o, "full-size block")
+    assert null_db(og, oc) <= NULL_DB
+    g.close()
