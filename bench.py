@@ -76,6 +76,8 @@ def main():
     ap.add_argument("--graph", action="store_true",
                     help="replay the block chain from a hipGraph (no per-pass events)")
     ap.add_argument("--seed", type=int, default=0x4B530002)
+    ap.add_argument("--waves-per-group", type=int, default=None)
+    ap.add_argument("--sort-digit-bits", type=int, default=None)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -87,6 +89,9 @@ def main():
 
     K, WU = args.steps, args.warmup
     cfg = W.config_c2(compensated=bool(args.compensated), profile=not args.graph)
+    for knob in ("waves_per_group", "sort_digit_bits"):
+        if getattr(args, knob) is not None:
+            cfg = cfg.replace(**{knob: getattr(args, knob)})
     seconds = (WU + K + 4) * cfg.block_frames / cfg.sample_rate
     t = time.time()
     _, bank, notes = W.workload_c2(args.seed, seconds=seconds, cfg=cfg,

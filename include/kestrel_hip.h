@@ -86,8 +86,7 @@ typedef struct kst_config {
     /* ---- MI355X knobs, no counterpart in the reference ---- */
     uint32_t form;                  /* 0 plain (reference-structured render), 1 wave64 */
     uint32_t compensated;           /* wave64 form: two-sum accumulation above the wave */
-    uint32_t waves_per_group;       /* wave64 form: waves per render workgroup (1..16) */
-    uint32_t groups_per_cu;         /* wave64 form: persistent render groups per CU */
+    uint32_t waves_per_group;       /* wave64 form: waves per render workgroup (1..4) */
     uint32_t sort_digit_bits;       /* radix width of the locality sort, 1..8 */
     int32_t device;                 /* HIP device ordinal */
     uint32_t profile;               /* per-pass timings (src/config.rs:337-338) */

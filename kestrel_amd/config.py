@@ -51,8 +51,7 @@ class Config:
     # ---- MI355X knobs (no counterpart in the reference) ----
     form: int = FORM_WAVE64
     compensated: bool = False
-    waves_per_group: int = 4
-    groups_per_cu: int = 3
+    waves_per_group: int = 1
     sort_digit_bits: int = 8
     device: int = 0
     profile: bool = False
@@ -122,7 +121,7 @@ class KstConfig(C.Structure):
         "max_steal_percent", "steal_fade_frames", "sort_voices", "interpolation",
         "decay_curve", "release_curve")] + [("env_floor", C.c_float)] + [
         (n, C.c_uint32) for n in ("max_param_variants", "kahan_reduce", "form",
-                                  "compensated", "waves_per_group", "groups_per_cu",
+                                  "compensated", "waves_per_group",
                                   "sort_digit_bits")] + [("device", C.c_int32),
                                                          ("profile", C.c_uint32)]
 

@@ -100,9 +100,8 @@ int validate(const kst_config &c) {
     if (c.decay_curve > 1 || c.release_curve > 1)
         return fail(KST_EINVAL, "envelope curve must be 0 (exponential) or 1 (linear)");
     if (c.max_param_variants == 0) return fail(KST_EINVAL, "max_param_variants must be non-zero");
-    if (c.waves_per_group == 0 || c.waves_per_group > 16)
-        return fail(KST_EINVAL, "waves_per_group %u must be in 1..=16", c.waves_per_group);
-    if (c.groups_per_cu == 0) return fail(KST_EINVAL, "groups_per_cu must be non-zero");
+    if (c.waves_per_group == 0 || c.waves_per_group > 4)
+        return fail(KST_EINVAL, "waves_per_group %u must be in 1..=4", c.waves_per_group);
     if (c.sort_digit_bits == 0 || c.sort_digit_bits > 8)
         return fail(KST_EINVAL, "sort_digit_bits %u must be in 1..=8", c.sort_digit_bits);
     // 32-bit SoA word offsets: field 23 of the last slot must stay below 2^32.
@@ -170,15 +169,17 @@ struct kst_backend {
     uint32_t max_steal = 0;
     SortPlan sort{};
     uint32_t sort_passes = 1, digit_bits = 1, tiles_max = 1;
-    uint32_t max_groups = 1;  // partial-buffer rows
+    uint32_t max_groups = 1;  // render workgroups at capacity (plain: partial-buffer width)
+    uint32_t max_rows = 1;    // wave64 partial rows at capacity (one per 64-voice batch)
 
     DevBuf<uint32_t> pools[2];
     DevBuf<int16_t> bank;
     uint32_t bank_len = 0;
     uint32_t bank_clamp = 1;
     DevBuf<kst_region_params> params;
-    DevBuf<float> partials;
-    DevBuf<uint32_t> keys[2], idx[2], sort_hist, sort_tot, sel_hist;
+    DevBuf<float> partials, rsum;
+    DevBuf<uint32_t> keys[2], idx[2], sort_hist, sort_tot, sel_cnt, cand_slot;
+    DevBuf<unsigned long long> cand_key, sel_min, sel_max;
     DevBuf<DevState> st;
 
     // Staged inputs (one slot for the per-block API, nblocks for staged runs).
@@ -204,6 +205,10 @@ struct kst_backend {
     float peak = 0.0f;
 
     bool profile = false;
+    // Select rounds launched per block. A round whose bin is already small
+    // enough returns at once; rounds still needed after these are finished by
+    // the final single-workgroup pass, so the count only affects speed.
+    uint32_t sel_rounds = 3;
     hipEvent_t ev[6] = {};
     std::vector<hipEvent_t> stage_ev;
     std::vector<kst_timing> timings;
@@ -221,11 +226,16 @@ struct kst_backend {
         bank.release();
         params.release();
         partials.release();
+        rsum.release();
         for (auto &k : keys) k.release();
         for (auto &k : idx) k.release();
         sort_hist.release();
         sort_tot.release();
-        sel_hist.release();
+        sel_cnt.release();
+        sel_min.release();
+        sel_max.release();
+        cand_slot.release();
+        cand_key.release();
         st.release();
         g_gates.release();
         g_chans.release();
@@ -280,8 +290,7 @@ struct kst_backend {
     uint32_t render_groups_for(uint32_t n) const {
         if (fast) {
             uint32_t waves = (n + 63) / 64;
-            uint32_t g = (waves + cfg.waves_per_group - 1) / cfg.waves_per_group;
-            return std::max(1u, std::min(g, max_groups));
+            return std::max(1u, (waves + cfg.waves_per_group - 1) / cfg.waves_per_group);
         }
         uint32_t cap = std::min(std::max(cfg.max_render_workgroups, 1u), 65535u);
         uint32_t g = (n + cfg.workgroup_size - 1) / cfg.workgroup_size;
@@ -323,6 +332,7 @@ struct kst_backend {
         a.exp_release = cfg.release_curve == 0;
         a.env_floor = cfg.env_floor;
         a.partials = partials.p;
+        a.rsum = rsum.p;
         a.render_groups = render_groups;
         a.waves_per_group = cfg.waves_per_group;
         a.plain_tile = cfg.reduce_tile;
@@ -343,7 +353,11 @@ struct kst_backend {
         a.phase_shift = sort.phase_shift;
         a.phase_mask = sort.phase_mask;
         a.dead_region = sort.dead_region;
-        a.sel_hist = sel_hist.p;
+        a.sel_cnt = sel_cnt.p;
+        a.sel_min = sel_min.p;
+        a.sel_max = sel_max.p;
+        a.cand_key = cand_key.p;
+        a.cand_slot = cand_slot.p;
         return a;
     }
 
@@ -358,17 +372,20 @@ struct kst_backend {
         hipStream_t s = stream;
         if (timed) (void)hipEventRecord(ev[0], s);
         launch_begin(a, s);
-        if (may_steal) launch_select(a, per_voice_grid(n_live), s);
+        if (may_steal)
+            launch_select(a, std::max(1u, std::min(std::min(cus, SEL_MAX_BLOCKS),
+                                                   (n_live + SEL_THREADS * 4 - 1) / (SEL_THREADS * 4))),
+                          sel_rounds, s);
         if (timed) (void)hipEventRecord(ev[1], s);
         if (spawn) launch_spawn(a, per_voice_grid(n_total), s);
         if (timed) (void)hipEventRecord(ev[2], s);
         if (fast)
-            launch_render_fast(a, chan, cfg.compensated != 0, s);
+            launch_render_fast(a, chan, s);
         else
             launch_render_plain(a, chan, s);
         if (timed) (void)hipEventRecord(ev[3], s);
         if (fast)
-            launch_reduce_fast(a, cfg.compensated != 0, s);
+            launch_reduce_fast(a, cfg.compensated != 0, (n_total + 63) / 64, s);
         else
             launch_reduce_plain(a, s);
         if (timed) (void)hipEventRecord(ev[4], s);
@@ -426,8 +443,7 @@ void kst_default_config(kst_config *c) {
     c->kahan_reduce = 0;
     c->form = 1;
     c->compensated = 0;
-    c->waves_per_group = 4;
-    c->groups_per_cu = 3;
+    c->waves_per_group = 1;
     c->sort_digit_bits = 8;
     c->device = 0;
     c->profile = 0;
@@ -477,15 +493,8 @@ int kst_create(const kst_config *cfg, const int16_t *pool, size_t pool_len,
     }
     b->tiles_max = (b->slots + SORT_TILE - 1) / SORT_TILE;
     if (b->fast) {
-        uint32_t waves = (b->slots + 63) / 64;
-        uint32_t g = (waves + c.waves_per_group - 1) / c.waves_per_group;
-        b->max_groups = std::max(1u, std::min(g, b->cus * c.groups_per_cu));
-        size_t lds = fast_render_lds_bytes(c.block_frames, c.waves_per_group, c.compensated != 0);
-        if (lds > 160 * 1024)
-            return cleanup(fail(KST_EINVAL,
-                                "block_frames %u with %u waves per group needs %zu bytes of LDS, "
-                                "over the 160 KiB of a CU; lower waves_per_group",
-                                c.block_frames, c.waves_per_group, lds));
+        b->max_rows = (b->slots + 63) / 64;
+        b->max_groups = (b->max_rows + c.waves_per_group - 1) / c.waves_per_group;
     } else {
         uint32_t cap = std::min(std::max(c.max_render_workgroups, 1u), 65535u);
         b->max_groups = std::max(1u, std::min(cap, (b->slots + c.workgroup_size - 1) / c.workgroup_size));
@@ -532,13 +541,19 @@ int kst_create(const kst_config *cfg, const int16_t *pool, size_t pool_len,
         if (b->keys[k].alloc(b->slots) != hipSuccess || b->idx[k].alloc(b->slots) != hipSuccess)
             return cleanup(fail(KST_EDEVICE, "allocating the sort buffers"));
     }
-    const size_t prow = b->fast ? two_f * (c.compensated ? 2 : 1) : two_f;
-    if (b->partials.alloc(prow * b->max_groups) != hipSuccess)
+    const size_t prows = b->fast ? b->max_rows : b->max_groups;
+    const size_t chunks = (b->max_rows + REDUCE_ROWS - 1) / REDUCE_ROWS;
+    if (b->partials.alloc(two_f * prows) != hipSuccess ||
+        b->rsum.alloc(two_f * 2 * std::max<size_t>(chunks, 1)) != hipSuccess)
         return cleanup(fail(KST_EDEVICE, "allocating the partial sums"));
     const uint32_t radix = 1u << b->digit_bits;
     if (b->sort_hist.alloc((size_t)b->sort_passes * radix * b->tiles_max) != hipSuccess ||
         b->sort_tot.alloc((size_t)b->sort_passes * radix) != hipSuccess ||
-        b->sel_hist.alloc(SEL_ROUNDS * 256) != hipSuccess || b->st.alloc(1) != hipSuccess)
+        b->sel_cnt.alloc(SEL_MAX_BLOCKS * SEL_BINS) != hipSuccess ||
+        b->sel_min.alloc(SEL_MAX_BLOCKS * SEL_BINS) != hipSuccess ||
+        b->sel_max.alloc(SEL_MAX_BLOCKS * SEL_BINS) != hipSuccess ||
+        b->cand_key.alloc(b->slots) != hipSuccess || b->cand_slot.alloc(b->slots) != hipSuccess ||
+        b->st.alloc(1) != hipSuccess)
         return cleanup(fail(KST_EDEVICE, "allocating the select/sort scratch"));
     DevState zero{};
     if (hipMemcpy(b->st.p, &zero, sizeof zero, hipMemcpyHostToDevice) != hipSuccess)

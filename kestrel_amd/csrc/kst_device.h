@@ -38,9 +38,19 @@ constexpr uint32_t CHAN_ACTIVE_BEND = 1, CHAN_ACTIVE_GAIN = 2, CHAN_ACTIVE_VARIA
 constexpr uint32_t STEAL_DROP_NEW = 1, STEAL_QUIETEST = 2;
 constexpr uint32_t ADMIT_EVEN = 1;
 
-// Radix select of the k-th smallest 64-bit steal key: eight MSB-first rounds
-// of one byte each (shaders/select.wgsl).
-constexpr uint32_t SEL_ROUNDS = 8;
+// Radix select of the k-th smallest 64-bit steal key (shaders/select.wgsl):
+// MSB-first rounds of SEL_BITS-wide digits. Each round also keeps the smallest
+// and largest key of every bin, so the next digit starts at the highest bit
+// that still varies inside the chosen bin (note-id keys skip their zero high
+// bits). Once the bin holds at most SEL_CAP keys they are ranked exactly in
+// LDS. Per-workgroup histograms are stored without atomics and summed by one
+// workgroup: the same global address hit from every CU is what costs.
+constexpr uint32_t SEL_BITS = 8;
+constexpr uint32_t SEL_BINS = 1u << SEL_BITS;
+constexpr uint32_t SEL_MAX_BLOCKS = 1024;
+constexpr uint32_t SEL_CAP = 4096;
+constexpr uint32_t SEL_THREADS = 1024;
+
 // Locality sort tiles: 256 threads x 8 items, processed in index order.
 constexpr uint32_t SORT_TPB = 256;
 constexpr uint32_t SORT_IPT = 8;
@@ -53,6 +63,8 @@ constexpr uint32_t MAX_SORT_PASSES = 32;
 constexpr uint32_t SUBTILE = 16;
 constexpr uint32_t TROW = 68;
 constexpr uint32_t TBUF_FLOATS = 2 * SUBTILE * TROW;  // 32 rows
+// Batch rows summed per workgroup in the first mixdown stage.
+constexpr uint32_t REDUCE_ROWS = 256;
 
 // Device-resident state of one backend, carried across blocks. The live count
 // lives here rather than in a kernel argument so a whole block of passes, or a
@@ -67,11 +79,14 @@ struct DevState {
     // Derived per block by k_begin (src/gpu/mod.rs:1015-1040).
     uint32_t want, steal_k, spawn_count, live_total;
     uint32_t cmd_off, chan_flags, live_new, peak_bits;
-    // Radix select: prefix and remaining rank entering each round.
-    uint32_t sel_prefix_hi[SEL_ROUNDS + 1];
-    uint32_t sel_prefix_lo[SEL_ROUNDS + 1];
-    uint32_t sel_k[SEL_ROUNDS + 1];
-    uint32_t thresh_hi, thresh_lo;
+    // Radix select state.
+    unsigned long long sel_prefix;  // bits >= sel_top of every remaining candidate
+    uint32_t sel_top;               // bits below this are still unresolved
+    uint32_t sel_k;                 // rank of the k-th key among the candidates
+    uint32_t sel_count;             // remaining candidates
+    uint32_t sel_done;              // candidates few enough to rank in LDS
+    uint32_t cand_n;                // candidates collected
+    uint32_t sel_pad;
 };
 
 // Per-block result row, what stats() reads after each render.
@@ -105,8 +120,9 @@ struct Args {
     uint32_t steal_rule, admit_rule, interp, exp_decay, exp_release;
     float env_floor;
     // Render / reduce.
-    float *partials;
-    uint32_t render_groups;  // rows of the partial buffer this block
+    float *partials;         // wave64: [batch][2F]; plain: [2F][nwg]
+    float *rsum;             // wave64 mixdown stage 1: [chunk][2F][hi, lo]
+    uint32_t render_groups;  // render workgroups launched this block
     uint32_t waves_per_group;
     uint32_t plain_tile, plain_wg, kahan;
     // Locality sort and compaction.
@@ -116,7 +132,11 @@ struct Args {
     uint32_t *sort_tot;   // [passes][radix]
     uint32_t sort_on, sort_passes, digit_bits, tiles_max;
     uint32_t region_shift, stage_shift, phase_shift, phase_mask, dead_region;
-    uint32_t *sel_hist;  // [SEL_ROUNDS][256]
+    uint32_t *sel_cnt;             // [SEL_MAX_BLOCKS][SEL_BINS] per-workgroup counts
+    unsigned long long *sel_min;   // [SEL_MAX_BLOCKS][SEL_BINS]
+    unsigned long long *sel_max;   // [SEL_MAX_BLOCKS][SEL_BINS]
+    unsigned long long *cand_key;  // [cap]
+    uint32_t *cand_slot;           // [cap]
 };
 
 // One of a pair of buffers, chosen without indexing the kernel-argument array
@@ -128,7 +148,7 @@ __device__ __forceinline__ T *sel2(T *const (&pair)[2], uint32_t i) {
 
 // ---- launchers (kst_pool.hip) ----
 void launch_begin(const Args &a, hipStream_t s);
-void launch_select(const Args &a, uint32_t grid, hipStream_t s);
+void launch_select(const Args &a, uint32_t grid, uint32_t rounds, hipStream_t s);
 void launch_spawn(const Args &a, uint32_t grid, hipStream_t s);
 void launch_sort(const Args &a, uint32_t grid, hipStream_t s);
 void launch_end(const Args &a, hipStream_t s);
@@ -136,9 +156,9 @@ void launch_end(const Args &a, hipStream_t s);
 // ---- launchers (kst_render.hip) ----
 // Returns false when the wave64 form does not cover this configuration.
 bool fast_render_supported(uint32_t F, uint32_t gate_frames, uint32_t interp);
-size_t fast_render_lds_bytes(uint32_t F, uint32_t waves_per_group, bool comp);
-void launch_render_fast(const Args &a, bool chan, bool comp, hipStream_t s);
-void launch_reduce_fast(const Args &a, bool comp, hipStream_t s);
+size_t fast_render_lds_bytes(uint32_t waves_per_group);
+void launch_render_fast(const Args &a, bool chan, hipStream_t s);
+void launch_reduce_fast(const Args &a, bool comp, uint32_t max_rows, hipStream_t s);
 void launch_render_plain(const Args &a, bool chan, hipStream_t s);
 void launch_reduce_plain(const Args &a, hipStream_t s);
 

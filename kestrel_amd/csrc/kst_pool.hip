@@ -1,8 +1,8 @@
 // kst_pool.hip -- the per-block pool passes around the render kernel:
 //
 //   k_begin      steal / spawn arithmetic of one block   src/gpu/mod.rs:1015-1040
-//   k_select     radix select of the k-th smallest key   shaders/select.wgsl
-//   k_mark       schedule the victims' stop frames       shaders/compact.wgsl:148-191
+//   k_sel_*      radix select of the steal_k smallest    shaders/select.wgsl
+//                keys, marking the victims' stop frames  shaders/compact.wgsl:148-191
 //   k_spawn      append the admitted note-ons            shaders/spawn.wgsl
 //   k_sort_*     stable LSD radix sort by locality key   shaders/sort.wgsl
 //                (or the 1-bit alive key: compaction)    shaders/compact.wgsl:26-146
@@ -33,8 +33,7 @@ __device__ __forceinline__ uint64_t steal_key(const Args &a, const uint32_t *poo
 __global__ void k_begin(Args a) {
     DevState *st = a.st;
     const uint32_t b = st->block;
-    // Clear the select histograms and sort totals for this block.
-    for (uint32_t i = threadIdx.x; i < SEL_ROUNDS * 256; i += blockDim.x) a.sel_hist[i] = 0;
+    // Clear the sort totals for this block.
     const uint32_t radix = 1u << a.digit_bits;
     for (uint32_t i = threadIdx.x; i < a.sort_passes * radix; i += blockDim.x) a.sort_tot[i] = 0;
     if (threadIdx.x != 0) return;
@@ -61,103 +60,297 @@ __global__ void k_begin(Args a) {
     st->peak_bits = 0;
     st->stolen += steal_k;
     st->dropped += raw - spawn;
-    st->sel_prefix_hi[0] = 0;
-    st->sel_prefix_lo[0] = 0;
-    st->sel_k[0] = steal_k > 0 ? (uint32_t)steal_k : 1;
+    st->sel_prefix = 0;
+    st->sel_top = 64;
+    st->sel_k = steal_k > 0 ? (uint32_t)steal_k : 1;
+    st->sel_count = (uint32_t)live;
+    st->sel_done = live <= SEL_CAP;  // few enough to rank all at once
+    st->cand_n = 0;
 }
 
-// Pick one byte of the answer from the previous round's histogram
-// (shaders/select.wgsl:235-262): the first bin whose inclusive count reaches k.
-// The whole workgroup scans the 256 bins in LDS; every workgroup computes the
-// same answer, and workgroup 0 publishes it for the next round.
-__device__ void refine(const Args &a, uint32_t r, uint64_t *s_prefix, uint32_t *s_k,
-                       uint32_t *scan) {
-    DevState *st = a.st;
-    const uint32_t prev = r - 1;
-    const uint32_t t = threadIdx.x;  // blockDim.x == 256
-    uint32_t h = a.sel_hist[prev * 256 + t];
-    scan[t] = h;
+// ---------------------------------------------------------------------------
+// Steal selection: the steal_k smallest keys, which are unique (the note id is
+// in every key), so "key <= the k-th smallest" names exactly steal_k voices --
+// the set src/cpu.rs:351-370 takes with select_nth_unstable.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long x) {
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long y = __shfl_xor(x, off);
+        x = y < x ? y : x;
+    }
+    return x;
+}
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long x) {
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long y = __shfl_xor(x, off);
+        x = y > x ? y : x;
+    }
+    return x;
+}
+
+__device__ __forceinline__ bool sel_match(unsigned long long key, unsigned long long prefix,
+                                          uint32_t top) {
+    return top >= 64 || (key >> top) == (prefix >> top);
+}
+
+// The window of the next digit: bits [lo, top).
+__device__ __forceinline__ uint32_t sel_lo(uint32_t top) {
+    return top > SEL_BITS ? top - SEL_BITS : 0;
+}
+
+// Count, smallest and largest key per bin of the candidates' next digit, into
+// per-bin LDS slots (cnt[], mn[], mx[]). Lanes sharing the wave's first digit
+// are folded into one update (neighbouring voices mostly do).
+__device__ __forceinline__ void sel_accumulate(bool match, unsigned long long key, uint32_t d,
+                                               uint32_t *cnt, unsigned long long *mn,
+                                               unsigned long long *mx) {
+    const uint64_t mm = __ballot(match);
+    if (!mm) return;
+    const uint32_t first = __ffsll((long long)mm) - 1;
+    const uint32_t d0 = __shfl(d, first);
+    const bool same = match && d == d0;
+    const uint64_t ms = __ballot(same);
+    const unsigned long long lo = wave_min_u64(same ? key : ~0ull);
+    const unsigned long long hi = wave_max_u64(same ? key : 0ull);
+    if ((threadIdx.x & 63) == first) {
+        atomicAdd(&cnt[d0], (uint32_t)__popcll(ms));
+        atomicMin(&mn[d0], lo);
+        atomicMax(&mx[d0], hi);
+    }
+    if (match && !same) {
+        atomicAdd(&cnt[d], 1u);
+        atomicMin(&mn[d], key);
+        atomicMax(&mx[d], key);
+    }
+}
+
+// Pick the bin holding the k-th key from `cnt/mn/mx` (256 entries, one per
+// thread of the calling workgroup) and narrow the state to it.
+__device__ void sel_choose(DevState *st, const uint32_t *cnt,
+                           const unsigned long long *mn, const unsigned long long *mx,
+                           uint32_t *scan) {
+    const uint32_t t = threadIdx.x;
+    const uint32_t c = t < SEL_BINS ? cnt[t] : 0;
+    if (t < SEL_BINS) scan[t] = c;
     __syncthreads();
-    for (uint32_t off = 1; off < 256; off <<= 1) {
-        uint32_t x = t >= off ? scan[t - off] : 0;
+    for (uint32_t off = 1; off < SEL_BINS; off <<= 1) {
+        uint32_t x = (t < SEL_BINS && t >= off) ? scan[t - off] : 0;
         __syncthreads();
-        scan[t] += x;
+        if (t < SEL_BINS) scan[t] += x;
         __syncthreads();
     }
-    const uint32_t kk = st->sel_k[prev];
-    const uint32_t incl = scan[t];
-    const uint32_t excl = incl - h;
-    if (incl >= kk && excl < kk) {
-        uint64_t p = ((uint64_t)st->sel_prefix_hi[prev] << 32) | st->sel_prefix_lo[prev];
-        p |= (uint64_t)t << (56 - 8 * prev);
-        *s_prefix = p;
-        *s_k = kk - excl;
-    }
-    __syncthreads();
-}
-
-__global__ void __launch_bounds__(256) k_select(Args a, uint32_t r) {
-    DevState *st = a.st;
-    if (st->steal_k == 0) return;
-    __shared__ uint32_t hist[256];
-    __shared__ uint64_t s_prefix;
-    __shared__ uint32_t s_k;
-    if (threadIdx.x == 0) {
-        // A histogram that never reaches k (it cannot, for unique keys) keeps
-        // bin 255, as the reference's refine does.
-        s_prefix = r == 0 ? 0ull : ~0ull;
-        s_k = st->sel_k[0];
-    }
-    __syncthreads();
-    if (r > 0) refine(a, r, &s_prefix, &s_k, hist);
-    if (threadIdx.x == 0 && blockIdx.x == 0 && r > 0) {
-        st->sel_prefix_hi[r] = (uint32_t)(s_prefix >> 32);
-        st->sel_prefix_lo[r] = (uint32_t)s_prefix;
-        st->sel_k[r] = s_k;
-    }
-    hist[threadIdx.x] = 0;
-    __syncthreads();
-    const uint64_t prefix = s_prefix;
-    const uint32_t *pool = sel2(a.pools, st->parity);
-    const uint32_t live = st->live;
-    const uint32_t shift = 56 - 8 * r;
-    const uint32_t hi_shift = 64 - 8 * r;  // bits above this round's byte
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < live;
-         i += gridDim.x * blockDim.x) {
-        uint64_t key = steal_key(a, pool, i);
-        bool match = r == 0 || (key >> hi_shift) == (prefix >> hi_shift);
-        if (match) atomicAdd(&hist[(key >> shift) & 255], 1u);
-    }
-    __syncthreads();
-    uint32_t c = hist[threadIdx.x];
-    if (c) atomicAdd(&a.sel_hist[r * 256 + threadIdx.x], c);
-}
-
-// The threshold is the k-th smallest key; keys are unique, so `key <= T` names
-// exactly steal_k voices (shaders/compact.wgsl:151-179, src/cpu.rs:365-370).
-__global__ void __launch_bounds__(256) k_mark(Args a) {
-    DevState *st = a.st;
-    if (st->steal_k == 0) return;
-    __shared__ uint64_t s_thresh;
-    __shared__ uint32_t s_k;
-    __shared__ uint32_t scan[256];
-    if (threadIdx.x == 0) s_thresh = ~0ull;
-    __syncthreads();
-    refine(a, SEL_ROUNDS, &s_thresh, &s_k, scan);
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        st->thresh_hi = (uint32_t)(s_thresh >> 32);
-        st->thresh_lo = (uint32_t)s_thresh;
-    }
-    const uint64_t t = s_thresh;
-    uint32_t *pool = sel2(a.pools, st->parity);
-    const uint32_t live = st->live;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < live;
-         i += gridDim.x * blockDim.x) {
-        if (steal_key(a, pool, i) <= t) {
-            uint32_t lo = pool[F_NOTE_LO * a.cap + i];
-            pool[F_STOP_REL * a.cap + i] = lo % a.steal_span + 1;
+    const uint32_t k = st->sel_k;
+    if (t < SEL_BINS) {
+        const uint32_t excl = scan[t] - c;
+        if (excl < k && excl + c >= k) {
+            // Keys of the bin lie in [mn, mx]: they share every bit above the
+            // highest one in which those two differ.
+            const unsigned long long diff = mn[t] ^ mx[t];
+            const uint32_t top = diff ? 64 - __clzll(diff) : 0;
+            st->sel_prefix = top >= 64 ? 0ull : (mn[t] >> top) << top;
+            st->sel_top = top;
+            st->sel_k = k - excl;
+            st->sel_count = c;
+            st->sel_done = c <= SEL_CAP || top == 0;
         }
     }
+    __syncthreads();
+}
+
+// One MSB-first round over the pool: per-workgroup bin statistics of the
+// candidates' next digit, stored without atomics.
+__global__ void __launch_bounds__(SEL_THREADS) k_sel_hist(Args a) {
+    DevState *st = a.st;
+    if (st->steal_k == 0 || st->sel_done) return;
+    __shared__ uint32_t cnt[SEL_BINS];
+    __shared__ unsigned long long mn[SEL_BINS], mx[SEL_BINS];
+    const uint32_t top = st->sel_top;
+    const unsigned long long prefix = st->sel_prefix;
+    const uint32_t lo = sel_lo(top);
+    const uint32_t dmask = (1u << (top - lo)) - 1;
+    for (uint32_t i = threadIdx.x; i < SEL_BINS; i += blockDim.x) {
+        cnt[i] = 0;
+        mn[i] = ~0ull;
+        mx[i] = 0;
+    }
+    __syncthreads();
+    const uint32_t *pool = sel2(a.pools, st->parity);
+    const uint32_t live = st->live;
+    const uint32_t lim = (live + 63) & ~63u;  // whole waves, so ballots see every lane
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < lim;
+         i += gridDim.x * blockDim.x) {
+        bool match = false;
+        unsigned long long key = 0;
+        uint32_t d = 0;
+        if (i < live) {
+            key = steal_key(a, pool, i);
+            match = sel_match(key, prefix, top);
+            d = (uint32_t)(key >> lo) & dmask;
+        }
+        sel_accumulate(match, key, d, cnt, mn, mx);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < SEL_BINS; i += blockDim.x) {
+        const size_t o = (size_t)blockIdx.x * SEL_BINS + i;
+        a.sel_cnt[o] = cnt[i];
+        a.sel_min[o] = mn[i];
+        a.sel_max[o] = mx[i];
+    }
+}
+
+// One workgroup sums the per-workgroup statistics and picks the bin.
+__global__ void __launch_bounds__(SEL_BINS) k_sel_refine(Args a, uint32_t blocks) {
+    DevState *st = a.st;
+    if (st->steal_k == 0 || st->sel_done) return;
+    __shared__ uint32_t cnt[SEL_BINS], scan[SEL_BINS];
+    __shared__ unsigned long long mn[SEL_BINS], mx[SEL_BINS];
+    const uint32_t t = threadIdx.x;
+    uint32_t c = 0;
+    unsigned long long lo = ~0ull, hi = 0;
+    for (uint32_t b = 0; b < blocks; ++b) {
+        const size_t o = (size_t)b * SEL_BINS + t;
+        c += a.sel_cnt[o];
+        const unsigned long long x = a.sel_min[o], y = a.sel_max[o];
+        lo = x < lo ? x : lo;
+        hi = y > hi ? y : hi;
+    }
+    cnt[t] = c;
+    mn[t] = lo;
+    mx[t] = hi;
+    __syncthreads();
+    sel_choose(st, cnt, mn, mx, scan);
+}
+
+__device__ __forceinline__ void mark_victim(const Args &a, uint32_t *pool, uint32_t i) {
+    // src/cpu.rs:365-369 / shaders/compact.wgsl:174-175
+    const uint32_t lo = pool[F_NOTE_LO * a.cap + i];
+    pool[F_STOP_REL * a.cap + i] = lo % a.steal_span + 1;
+}
+
+// Every key below the candidates is a victim; the candidates are collected
+// (one global reservation per workgroup).
+__global__ void __launch_bounds__(SEL_THREADS) k_sel_collect(Args a) {
+    DevState *st = a.st;
+    if (st->steal_k == 0) return;
+    __shared__ uint32_t s_n, s_base;
+    uint32_t *pool = sel2(a.pools, st->parity);
+    const uint32_t live = st->live;
+    const uint32_t top = st->sel_top;
+    const unsigned long long prefix = st->sel_prefix;
+    const unsigned long long pb = top >= 64 ? 0ull : prefix >> top;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < live;
+         i += gridDim.x * blockDim.x) {
+        const unsigned long long kb = top >= 64 ? 0ull : steal_key(a, pool, i) >> top;
+        if (kb < pb) mark_victim(a, pool, i);
+        else if (kb == pb) atomicAdd(&s_n, 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) s_base = s_n ? atomicAdd(&st->cand_n, s_n) : 0;
+    __syncthreads();
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < live;
+         i += gridDim.x * blockDim.x) {
+        const unsigned long long key = steal_key(a, pool, i);
+        if ((top >= 64 ? 0ull : key >> top) == pb) {
+            const uint32_t at = s_base + atomicAdd(&s_n, 1u);
+            a.cand_key[at] = key;
+            a.cand_slot[at] = i;
+        }
+    }
+}
+
+// Finish on the candidates: any rounds still needed (one workgroup, rarely
+// taken), then an exact rank in LDS; the first k are victims.
+__global__ void __launch_bounds__(SEL_THREADS) k_sel_final(Args a) {
+    DevState *st = a.st;
+    if (st->steal_k == 0) return;
+    __shared__ unsigned long long key[SEL_CAP];
+    __shared__ uint32_t slot[SEL_CAP];
+    __shared__ uint32_t cnt[SEL_BINS], scan[SEL_BINS];
+    __shared__ unsigned long long mn[SEL_BINS], mx[SEL_BINS];
+    __shared__ uint32_t s_n;
+    uint32_t *pool = sel2(a.pools, st->parity);
+    uint32_t n = st->cand_n;
+    // Each round lowers sel_top by at least SEL_BITS, so 64 / SEL_BITS rounds
+    // always end it; the bound only guards against a broken state.
+    for (uint32_t round = 0; round <= 64 / SEL_BITS && !st->sel_done; ++round) {
+        // Another round over the collected candidates.
+        const uint32_t top = st->sel_top;
+        const uint32_t lo = sel_lo(top);
+        const uint32_t dmask = (1u << (top - lo)) - 1;
+        for (uint32_t i = threadIdx.x; i < SEL_BINS; i += blockDim.x) {
+            cnt[i] = 0;
+            mn[i] = ~0ull;
+            mx[i] = 0;
+        }
+        __syncthreads();
+        const uint32_t lim = (n + 63) & ~63u;
+        for (uint32_t i = threadIdx.x; i < lim; i += blockDim.x) {
+            const bool in = i < n;
+            const unsigned long long k = in ? a.cand_key[i] : 0ull;
+            sel_accumulate(in, k, in ? (uint32_t)(k >> lo) & dmask : 0u, cnt, mn, mx);
+        }
+        __syncthreads();
+        sel_choose(st, cnt, mn, mx, scan);
+        // Keep the chosen bin's candidates; those below it are victims.
+        const uint32_t ntop = st->sel_top;
+        const unsigned long long pb = ntop >= 64 ? 0ull : st->sel_prefix >> ntop;
+        if (threadIdx.x == 0) s_n = 0;
+        __syncthreads();
+        for (uint32_t base = 0; base < n; base += blockDim.x) {
+            const uint32_t i = base + threadIdx.x;
+            unsigned long long k = 0;
+            uint32_t sl = 0;
+            bool keep = false;
+            if (i < n) {
+                k = a.cand_key[i];
+                sl = a.cand_slot[i];
+                const unsigned long long kb = ntop >= 64 ? 0ull : k >> ntop;
+                if (kb < pb) mark_victim(a, pool, sl);
+                keep = kb == pb;
+            }
+            __syncthreads();  // every read of this chunk before any write
+            if (keep) {
+                const uint32_t at = atomicAdd(&s_n, 1u);
+                a.cand_key[at] = k;
+                a.cand_slot[at] = sl;
+            }
+            __syncthreads();
+        }
+        n = s_n;
+        __syncthreads();
+    }
+    if (n > SEL_CAP) n = SEL_CAP;  // cannot happen: done means n <= SEL_CAP or one key
+    const uint32_t k = st->sel_k;
+    for (uint32_t i = threadIdx.x; i < SEL_CAP; i += blockDim.x) {
+        key[i] = i < n ? a.cand_key[i] : ~0ull;
+        slot[i] = i < n ? a.cand_slot[i] : 0;
+    }
+    __syncthreads();
+    for (uint32_t size = 2; size <= SEL_CAP; size <<= 1) {
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            for (uint32_t i = threadIdx.x; i < SEL_CAP; i += blockDim.x) {
+                const uint32_t j = i ^ stride;
+                if (j > i) {
+                    const bool up = (i & size) == 0;
+                    const unsigned long long x = key[i], y = key[j];
+                    if ((x > y) == up) {
+                        key[i] = y;
+                        key[j] = x;
+                        const uint32_t tt = slot[i];
+                        slot[i] = slot[j];
+                        slot[j] = tt;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (uint32_t i = threadIdx.x; i < k && i < n; i += blockDim.x) mark_victim(a, pool, slot[i]);
 }
 
 // shaders/spawn.wgsl:35-78; thinning per src/voice.rs:72-75 when the block is
@@ -381,9 +574,13 @@ __global__ void k_end(Args a) {
 
 void launch_begin(const Args &a, hipStream_t s) { k_begin<<<1, 256, 0, s>>>(a); }
 
-void launch_select(const Args &a, uint32_t grid, hipStream_t s) {
-    for (uint32_t r = 0; r < SEL_ROUNDS; ++r) k_select<<<grid, 256, 0, s>>>(a, r);
-    k_mark<<<grid, 256, 0, s>>>(a);
+void launch_select(const Args &a, uint32_t grid, uint32_t rounds, hipStream_t s) {
+    for (uint32_t r = 0; r < rounds; ++r) {
+        k_sel_hist<<<grid, SEL_THREADS, 0, s>>>(a);
+        k_sel_refine<<<1, SEL_BINS, 0, s>>>(a, grid);
+    }
+    k_sel_collect<<<grid, SEL_THREADS, 0, s>>>(a);
+    k_sel_final<<<1, SEL_THREADS, 0, s>>>(a);
 }
 
 void launch_spawn(const Args &a, uint32_t grid, hipStream_t s) {

@@ -378,33 +378,63 @@ __device__ __forceinline__ bool fast_ok(const Args &a, const Voice &v, uint32_t 
     return true;
 }
 
+// One 32-bit word holding the int16 frames at byte offsets off and off + 2
+// (gfx950 global loads take 2-byte-aligned addresses).
+__device__ __forceinline__ uint32_t tap_pair(const char *bank, uint32_t byte_off) {
+    typedef uint32_t u32_a2 __attribute__((aligned(2)));
+    return *reinterpret_cast<const u32_a2 *>(bank + byte_off);
+}
+
+// int16 -> f32 as src/cpu.rs:190-194 does it: times 1/32767, clamped at -1.
+// Without a -32768 in the bank the clamp cannot change anything.
 template <bool CLAMP>
-__device__ __forceinline__ float tap(const char *bank, uint32_t byte_off) {
-    const int x = *reinterpret_cast<const int16_t *>(bank + byte_off);
-    const float f = (float)x * INV_32767;
+__device__ __forceinline__ float norm_lo(uint32_t w) {
+    const float f = (float)(int)(int16_t)(w & 0xFFFFu) * INV_32767;
+    return CLAMP ? fmaxf(f, -1.0f) : f;
+}
+template <bool CLAMP>
+__device__ __forceinline__ float norm_hi(uint32_t w) {
+    const float f = (float)((int)w >> 16) * INV_32767;
     return CLAMP ? fmaxf(f, -1.0f) : f;
 }
 
-// FILT: 0 every live lane unfiltered, 1 every live lane filtered, 2 mixed.
-template <int INTERP, bool CLAMP, int FILT>
-__device__ __forceinline__ void fast_subtile(const Args &a, Voice &v, float *tb, uint32_t lane,
-                                             bool env_static) {
+// FILT: 0 every sounding lane unfiltered, 1 every sounding lane filtered,
+// 2 mixed. ENV_STATIC: every sounding lane sustains through the sub-tile.
+template <int INTERP, bool CLAMP, int FILT, bool ENV_STATIC>
+__device__ __forceinline__ void fast_subtile(const Args &a, Voice &v, float *tb, uint32_t lane) {
     if (!v.live || v.stage == ENV_DEAD) {
         v.gl = v.gr = 0.0f;
         v.step = 0;
         v.phase = (uint64_t)(INTERP == 2 ? 1u : 0u) << 32;
         v.base = 0;
     }
-    float m, c, thi, tlo;
-    env_coefs(a, v, m, c, thi, tlo);
     const char *bank = reinterpret_cast<const char *>(a.bank);
     const uint32_t base2 = v.base * 2u;
+    // Every frame's taps first: the phase chain is integer arithmetic, so all
+    // of the sub-tile's loads can be in flight before the first is needed.
+    uint32_t w0[SUBTILE], w1[INTERP == 2 ? SUBTILE : 1];
+    {
+        uint64_t ph = v.phase;
+#pragma unroll
+        for (uint32_t i = 0; i < SUBTILE; ++i) {
+            const uint32_t off = base2 + (uint32_t)(ph >> 32) * 2u;
+            if (INTERP == 2) {
+                w0[i] = tap_pair(bank, off - 2);  // frames idx-1, idx
+                w1[i] = tap_pair(bank, off + 2);  // frames idx+1, idx+2
+            } else {
+                w0[i] = tap_pair(bank, off);      // frames idx, idx+1
+            }
+            ph += v.step;
+        }
+    }
+    float m, c, thi, tlo;
+    env_coefs(a, v, m, c, thi, tlo);
     const bool filt = FILT == 1 || (FILT == 2 && (v.p.flags & RP_FILTER));
     float g = fminf(v.level, 1.0f);
     bool kill = false;
 #pragma unroll
     for (uint32_t i = 0; i < SUBTILE; ++i) {
-        if (!env_static) {
+        if (!ENV_STATIC) {
             if (kill) {
                 // A lane that died on the previous frame is silent from here.
                 if (v.stage == ENV_DEAD) v.gl = v.gr = 0.0f;
@@ -421,23 +451,21 @@ __device__ __forceinline__ void fast_subtile(const Args &a, Voice &v, float *tb,
             }
             g = fminf(v.level, 1.0f);
         }
-        const uint32_t idx = (uint32_t)(v.phase >> 32);
-        const uint32_t off = base2 + idx * 2u;
         const float flo = (float)(uint32_t)v.phase;
         float s;
         if (INTERP == 0) {
-            s = tap<CLAMP>(bank, off);
+            s = norm_lo<CLAMP>(w0[i]);
         } else if (INTERP == 1) {
-            const float s0 = tap<CLAMP>(bank, off);
-            const float s1 = tap<CLAMP>(bank, off + 2);
+            const float s0 = norm_lo<CLAMP>(w0[i]);
+            const float s1 = norm_hi<CLAMP>(w0[i]);
             // s0 + (s1 - s0) * (flo * 2^-32): the power-of-two scale commutes
             // with rounding, so one fused step gives the same bits.
             s = fmaf((s1 - s0) * flo, FRAC_SCALE, s0);
         } else {
-            const float sm1 = tap<CLAMP>(bank, off - 2);
-            const float s0 = tap<CLAMP>(bank, off);
-            const float s1 = tap<CLAMP>(bank, off + 2);
-            const float s2 = tap<CLAMP>(bank, off + 4);
+            const float sm1 = norm_lo<CLAMP>(w0[i]);
+            const float s0 = norm_hi<CLAMP>(w0[i]);
+            const float s1 = norm_lo<CLAMP>(w1[i]);
+            const float s2 = norm_hi<CLAMP>(w1[i]);
             const float frac = flo * FRAC_SCALE;
             const float ca = -0.5f * sm1 + 1.5f * s0 - 1.5f * s1 + 0.5f * s2;
             const float cb = sm1 - 2.5f * s0 + 2.0f * s1 - 0.5f * s2;
@@ -456,7 +484,7 @@ __device__ __forceinline__ void fast_subtile(const Args &a, Voice &v, float *tb,
         tb[(2 * i + 1) * TROW + lane] = y * v.gr;
         v.phase += v.step;
     }
-    if (kill && v.stage == ENV_DEAD) v.gl = v.gr = 0.0f;
+    if (!ENV_STATIC && kill && v.stage == ENV_DEAD) v.gl = v.gr = 0.0f;
 }
 
 template <int INTERP, bool CLAMP>
@@ -466,12 +494,14 @@ __device__ __forceinline__ void fast_subtile_dispatch(const Args &a, Voice &v, f
     const bool env_static = !__any(sounding && v.stage != ENV_SUSTAIN);
     const bool f = (v.p.flags & RP_FILTER) != 0;
     if (!__any(sounding && !f)) {
-        env_static ? fast_subtile<INTERP, CLAMP, 1>(a, v, tb, lane, true)
-                   : fast_subtile<INTERP, CLAMP, 1>(a, v, tb, lane, false);
+        if (env_static)
+            fast_subtile<INTERP, CLAMP, 1, true>(a, v, tb, lane);
+        else
+            fast_subtile<INTERP, CLAMP, 1, false>(a, v, tb, lane);
     } else if (!__any(sounding && f)) {
-        fast_subtile<INTERP, CLAMP, 0>(a, v, tb, lane, env_static);
+        fast_subtile<INTERP, CLAMP, 0, false>(a, v, tb, lane);
     } else {
-        fast_subtile<INTERP, CLAMP, 2>(a, v, tb, lane, env_static);
+        fast_subtile<INTERP, CLAMP, 2, false>(a, v, tb, lane);
     }
 }
 
@@ -486,138 +516,100 @@ __device__ __forceinline__ void two_sum(float a, float b, float &s, float &e) {
 // wave64 form
 // ---------------------------------------------------------------------------
 
-// Per-wave LDS: the 32 x 68 transposition buffer, then 2F accumulated output
-// samples (and 2F compensation terms).
-__host__ __device__ constexpr uint32_t wave_lds_floats(uint32_t F, bool comp) {
-    return TBUF_FLOATS + 2 * F * (comp ? 2 : 1);
-}
-
-template <int INTERP, bool CHAN, bool COMP>
-__global__ void __launch_bounds__(1024) k_render_fast(Args a) {
+// One wave renders one batch of 64 voices for the whole block. Each 16-frame
+// sub-tile it reduces its 64 x 32 contributions through LDS and writes the 32
+// sums into its own row of the partial buffer (row = batch), so no two waves
+// share an accumulator and the summation order never depends on scheduling.
+template <int INTERP, bool CHAN, bool CLAMP>
+__global__ void __launch_bounds__(256) k_render_fast(Args a) {
     extern __shared__ float lds[];
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t wpg = blockDim.x >> 6;
     const uint32_t F = a.F;
     const uint32_t two_f = 2 * F;
-    float *tb = lds + wave * wave_lds_floats(F, COMP);
-    float *acc = tb + TBUF_FLOATS;
-    float *accl = acc + two_f;
+    float *tb = lds + wave * TBUF_FLOATS;
 
     DevState *st = a.st;
     const uint32_t live_total = st->live_total;
+    const uint32_t batch = blockIdx.x * wpg + wave;
+    if (batch * 64 >= live_total) return;
     const uint32_t b = st->block;
     const uint32_t chan_flags = CHAN ? st->chan_flags : 0u;
     uint32_t *pool = sel2(a.pools, st->parity);
     const uint32_t *gates = a.gates + (size_t)b * a.gate_words;
     const uint32_t *chan = a.chans + (size_t)b * a.chan_words;
-
-    for (uint32_t j = lane; j < two_f; j += 64) {
-        acc[j] = 0.0f;
-        if (COMP) accl[j] = 0.0f;
-    }
+    float *prow = a.partials + (size_t)batch * two_f;
 
     const uint32_t half = lane >> 5;  // which 32 voices this lane sums over
     const uint32_t o = lane & 31;     // which output of the sub-tile it owns
-    const uint32_t nw = gridDim.x * wpg;
-    for (uint32_t batch = blockIdx.x * wpg + wave; batch * 64 < live_total; batch += nw) {
-        const uint32_t vi = batch * 64 + lane;
-        Voice v;
-        load_voice(a, pool, vi, vi < live_total, v);
-        for (uint32_t s = 0; s < F / SUBTILE; ++s) {
-            const uint32_t f0 = s * SUBTILE;
-            if (f0 % a.gate_frames == 0) gate_tile<CHAN>(a, gates, chan, chan_flags, f0 / a.gate_frames, v);
-            if (!__any(!fast_ok<INTERP>(a, v, f0))) {
-                if (a.clamp)
-                    fast_subtile_dispatch<INTERP, true>(a, v, tb, lane);
-                else
-                    fast_subtile_dispatch<INTERP, false>(a, v, tb, lane);
-            } else {
-                for (uint32_t i = 0; i < SUBTILE; ++i) {
-                    float ol, orr;
-                    frame<INTERP>(a, f0 + i, v, ol, orr);
-                    tb[(2 * i) * TROW + lane] = ol;
-                    tb[(2 * i + 1) * TROW + lane] = orr;
-                }
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            // Transposed read: output o over the 32 voices of this half.
-            const float4 *row = reinterpret_cast<const float4 *>(tb + o * TROW + 32 * half);
-            float4 q[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) q[k] = row[k];
-            float t[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) t[k] = (q[k].x + q[k].y) + (q[k].z + q[k].w);
-            const float sum = ((t[0] + t[1]) + (t[2] + t[3])) + ((t[4] + t[5]) + (t[6] + t[7]));
-            const float other = __shfl_xor(sum, 32);
-            // Same value in both halves: (h0 + h1) == (h1 + h0).
-            const float total = half ? other + sum : sum + other;
-            // Lanes of half h own output 32*s + o; accumulate where s%2 == h so
-            // each output sample is added by exactly one lane.
-            if ((s & 1) == half) {
-                const uint32_t j = s * 32 + o;
-                if (COMP) {
-                    float hi, e;
-                    two_sum(acc[j], total, hi, e);
-                    acc[j] = hi;
-                    accl[j] = accl[j] + e;
-                } else {
-                    acc[j] = acc[j] + total;
-                }
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        }
-        if (v.live) store_voice(a, pool, vi, v);
-    }
-
-    // Workgroup combine in wave order, then one partial row per workgroup.
-    __syncthreads();
-    float *prow = a.partials + (size_t)blockIdx.x * two_f * (COMP ? 2 : 1);
-    for (uint32_t j = threadIdx.x; j < two_f; j += blockDim.x) {
-        float hi = lds[TBUF_FLOATS + j];
-        float lo = COMP ? lds[TBUF_FLOATS + two_f + j] : 0.0f;
-        for (uint32_t w = 1; w < wpg; ++w) {
-            const float *wl = lds + w * wave_lds_floats(F, COMP) + TBUF_FLOATS;
-            if (COMP) {
-                float s, e;
-                two_sum(hi, wl[j], s, e);
-                hi = s;
-                lo = lo + (e + wl[two_f + j]);
-            } else {
-                hi = hi + wl[j];
+    const uint32_t vi = batch * 64 + lane;
+    Voice v;
+    load_voice(a, pool, vi, vi < live_total, v);
+    for (uint32_t s = 0; s < F / SUBTILE; ++s) {
+        const uint32_t f0 = s * SUBTILE;
+        if (f0 % a.gate_frames == 0) gate_tile<CHAN>(a, gates, chan, chan_flags, f0 / a.gate_frames, v);
+        if (!__any(!fast_ok<INTERP>(a, v, f0))) {
+            fast_subtile_dispatch<INTERP, CLAMP>(a, v, tb, lane);
+        } else {
+#pragma unroll 1
+            for (uint32_t i = 0; i < SUBTILE; ++i) {
+                float ol, orr;
+                frame<INTERP>(a, f0 + i, v, ol, orr);
+                tb[(2 * i) * TROW + lane] = ol;
+                tb[(2 * i + 1) * TROW + lane] = orr;
             }
         }
-        prow[j] = hi;
-        if (COMP) prow[two_f + j] = lo;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // Transposed read: output o over the 32 voices of this half.
+        const float4 *row = reinterpret_cast<const float4 *>(tb + o * TROW + 32 * half);
+        float4 q[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) q[k] = row[k];
+        float t[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) t[k] = (q[k].x + q[k].y) + (q[k].z + q[k].w);
+        const float sum = ((t[0] + t[1]) + (t[2] + t[3])) + ((t[4] + t[5]) + (t[6] + t[7]));
+        const float other = __shfl_xor(sum, 32);
+        // Same value in both halves: (h0 + h1) == (h1 + h0). Half s%2 stores it.
+        const float total = half ? other + sum : sum + other;
+        if ((s & 1) == half) prow[s * 32 + o] = total;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
+    if (v.live) store_voice(a, pool, vi, v);
 }
 
-// Column sums of the partial rows: out[j] = sum over workgroup rows, in row
-// order, 64 columns per workgroup and 4 row phases per column.
+// Stage 1 of the mixdown: column sums of the batch rows in fixed order, one
+// chunk of REDUCE_ROWS rows per workgroup (64 columns x 4 row phases), written
+// as (hi, lo) pairs. With COMP the sums are compensated (two-sum), so the
+// cross-batch sum of a million voices keeps its low-order bits.
 template <bool COMP>
-__global__ void __launch_bounds__(256) k_reduce_fast(Args a) {
+__global__ void __launch_bounds__(256) k_reduce_rows(Args a) {
     DevState *st = a.st;
     const uint32_t two_f = 2 * a.F;
-    const uint32_t rows = a.render_groups;
+    const uint32_t rows = (st->live_total + 63) / 64;
+    const uint32_t chunk = blockIdx.y;
+    const uint32_t r0 = chunk * REDUCE_ROWS;
+    if (r0 >= rows) return;
+    const uint32_t r1 = r0 + REDUCE_ROWS < rows ? r0 + REDUCE_ROWS : rows;
     const uint32_t col = blockIdx.x * 64 + (threadIdx.x & 63);
     const uint32_t ph = threadIdx.x >> 6;
     __shared__ float sh[2][4][64];
     float hi = 0.0f, lo = 0.0f;
     if (col < two_f) {
-        for (uint32_t r = ph; r < rows; r += 4) {
-            const float *row = a.partials + (size_t)r * two_f * (COMP ? 2 : 1);
+        for (uint32_t r = r0 + ph; r < r1; r += 4) {
+            const float x = a.partials[(size_t)r * two_f + col];
             if (COMP) {
                 float s, e;
-                two_sum(hi, row[col], s, e);
+                two_sum(hi, x, s, e);
                 hi = s;
-                lo = lo + (e + row[two_f + col]);
+                lo = lo + e;
             } else {
-                hi = hi + row[col];
+                hi = hi + x;
             }
         }
     }
@@ -636,11 +628,37 @@ __global__ void __launch_bounds__(256) k_reduce_fast(Args a) {
                 h = h + sh[0][k][threadIdx.x];
             }
         }
-        const float v = COMP ? h + l : h;
-        a.out[(size_t)st->block * two_f + col] = v;
-        const float m = fabsf(v);
-        if (m == m) atomicMax(&st->peak_bits, __float_as_uint(m));
+        a.rsum[((size_t)chunk * two_f + col) * 2] = h;
+        a.rsum[((size_t)chunk * two_f + col) * 2 + 1] = l;
     }
+}
+
+// Stage 2: the chunks in order, one thread per output sample, then the block
+// peak (src/cpu.rs:615-625).
+template <bool COMP>
+__global__ void __launch_bounds__(256) k_reduce_final(Args a) {
+    DevState *st = a.st;
+    const uint32_t two_f = 2 * a.F;
+    const uint32_t col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= two_f) return;
+    const uint32_t rows = (st->live_total + 63) / 64;
+    const uint32_t chunks = (rows + REDUCE_ROWS - 1) / REDUCE_ROWS;
+    float h = 0.0f, l = 0.0f;
+    for (uint32_t c = 0; c < chunks; ++c) {
+        const float x = a.rsum[((size_t)c * two_f + col) * 2];
+        if (COMP) {
+            float s, e;
+            two_sum(h, x, s, e);
+            h = s;
+            l = l + (e + a.rsum[((size_t)c * two_f + col) * 2 + 1]);
+        } else {
+            h = h + x;
+        }
+    }
+    const float v = COMP ? h + l : h;
+    a.out[(size_t)st->block * two_f + col] = v;
+    const float m = fabsf(v);
+    if (m == m) atomicMax(&st->peak_bits, __float_as_uint(m));
 }
 
 // ---------------------------------------------------------------------------
@@ -737,17 +755,11 @@ __global__ void __launch_bounds__(1024) k_reduce_plain(Args a) {
     }
 }
 
-template <int INTERP, bool CHAN, bool COMP>
+template <int INTERP, bool CHAN, bool CLAMP>
 void launch_fast_t(const Args &a, hipStream_t s) {
     const uint32_t threads = a.waves_per_group * 64;
-    const size_t lds = (size_t)a.waves_per_group * wave_lds_floats(a.F, COMP) * sizeof(float);
-    if (lds > 65536) {
-        // gfx950 gives a workgroup up to 160 KiB of LDS; past 64 KiB the
-        // kernel has to ask for it.
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_render_fast<INTERP, CHAN, COMP>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    }
-    k_render_fast<INTERP, CHAN, COMP><<<a.render_groups, threads, lds, s>>>(a);
+    const size_t lds = (size_t)a.waves_per_group * TBUF_FLOATS * sizeof(float);
+    k_render_fast<INTERP, CHAN, CLAMP><<<a.render_groups, threads, lds, s>>>(a);
 }
 
 template <int INTERP, bool CHAN>
@@ -763,33 +775,36 @@ bool fast_render_supported(uint32_t F, uint32_t gate_frames, uint32_t interp) {
     return F % 32 == 0 && gate_frames % SUBTILE == 0 && interp <= 2;
 }
 
-size_t fast_render_lds_bytes(uint32_t F, uint32_t waves_per_group, bool comp) {
-    return (size_t)waves_per_group * wave_lds_floats(F, comp) * sizeof(float);
+size_t fast_render_lds_bytes(uint32_t waves_per_group) {
+    return (size_t)waves_per_group * TBUF_FLOATS * sizeof(float);
 }
 
-void launch_render_fast(const Args &a, bool chan, bool comp, hipStream_t s) {
-#define KST_FAST(I)                                                        \
-    if (a.interp == I) {                                                   \
-        if (chan) {                                                        \
-            comp ? launch_fast_t<I, true, true>(a, s) : launch_fast_t<I, true, false>(a, s); \
-        } else {                                                           \
-            comp ? launch_fast_t<I, false, true>(a, s) : launch_fast_t<I, false, false>(a, s); \
-        }                                                                  \
-        return;                                                            \
+void launch_render_fast(const Args &a, bool chan, hipStream_t s) {
+#define KST_FAST(I, CL)                                                                    \
+    if (a.interp == I && (a.clamp != 0) == CL) {                                           \
+        chan ? launch_fast_t<I, true, CL>(a, s) : launch_fast_t<I, false, CL>(a, s);       \
+        return;                                                                            \
     }
-    KST_FAST(0)
-    KST_FAST(1)
-    KST_FAST(2)
+    KST_FAST(0, false)
+    KST_FAST(1, false)
+    KST_FAST(2, false)
+    KST_FAST(0, true)
+    KST_FAST(1, true)
+    KST_FAST(2, true)
 #undef KST_FAST
 }
 
-void launch_reduce_fast(const Args &a, bool comp, hipStream_t s) {
-    const uint32_t cols = 2 * a.F;
-    const uint32_t grid = (cols + 63) / 64;
-    if (comp)
-        k_reduce_fast<true><<<grid, 256, 0, s>>>(a);
-    else
-        k_reduce_fast<false><<<grid, 256, 0, s>>>(a);
+void launch_reduce_fast(const Args &a, bool comp, uint32_t max_rows, hipStream_t s) {
+    const uint32_t two_f = 2 * a.F;
+    const dim3 g1((two_f + 63) / 64, (max_rows + REDUCE_ROWS - 1) / REDUCE_ROWS);
+    const uint32_t g2 = (two_f + 255) / 256;
+    if (comp) {
+        k_reduce_rows<true><<<g1, 256, 0, s>>>(a);
+        k_reduce_final<true><<<g2, 256, 0, s>>>(a);
+    } else {
+        k_reduce_rows<false><<<g1, 256, 0, s>>>(a);
+        k_reduce_final<false><<<g2, 256, 0, s>>>(a);
+    }
 }
 
 void launch_render_plain(const Args &a, bool chan, hipStream_t s) {

@@ -44,7 +44,7 @@ def test_struct_layouts():
     assert abi.PARAMS_DTYPE.itemsize == 48
     assert C.sizeof(abi.KstBlockStats) == 32
     from kestrel_amd.config import KstConfig
-    assert C.sizeof(KstConfig) == 25 * 4
+    assert C.sizeof(KstConfig) == 24 * 4
 
 
 def test_default_config_mirrors_the_reference(lib):
