@@ -52,7 +52,7 @@ class Config:
     form: int = FORM_WAVE64
     compensated: bool = False
     waves_per_group: int = 1
-    sort_digit_bits: int = 8
+    sort_digit_bits: int = 9
     device: int = 0
     profile: bool = False
 

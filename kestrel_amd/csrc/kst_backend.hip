@@ -102,8 +102,9 @@ int validate(const kst_config &c) {
     if (c.max_param_variants == 0) return fail(KST_EINVAL, "max_param_variants must be non-zero");
     if (c.waves_per_group == 0 || c.waves_per_group > 4)
         return fail(KST_EINVAL, "waves_per_group %u must be in 1..=4", c.waves_per_group);
-    if (c.sort_digit_bits == 0 || c.sort_digit_bits > 8)
-        return fail(KST_EINVAL, "sort_digit_bits %u must be in 1..=8", c.sort_digit_bits);
+    if (c.sort_digit_bits == 0 || c.sort_digit_bits > SORT_MAX_BITS)
+        return fail(KST_EINVAL, "sort_digit_bits %u must be in 1..=%u", c.sort_digit_bits,
+                    SORT_MAX_BITS);
     // 32-bit SoA word offsets: field 23 of the last slot must stay below 2^32.
     if ((uint64_t)cfg_pool_slots(c) * VOICE_FIELDS >= (1ull << 32))
         return fail(KST_EINVAL,
@@ -177,7 +178,8 @@ struct kst_backend {
     uint32_t bank_len = 0;
     uint32_t bank_clamp = 1;
     DevBuf<kst_region_params> params;
-    DevBuf<float> partials, rsum;
+    DevBuf<float> partials, rsum, fade_tab;
+    uint32_t fade_lds = 0;
     DevBuf<uint32_t> keys[2], idx[2], sort_hist, sort_tot, sel_cnt, cand_slot;
     DevBuf<unsigned long long> cand_key, sel_min, sel_max;
     DevBuf<DevState> st;
@@ -227,6 +229,7 @@ struct kst_backend {
         params.release();
         partials.release();
         rsum.release();
+        fade_tab.release();
         for (auto &k : keys) k.release();
         for (auto &k : idx) k.release();
         sort_hist.release();
@@ -332,6 +335,8 @@ struct kst_backend {
         a.exp_release = cfg.release_curve == 0;
         a.env_floor = cfg.env_floor;
         a.partials = partials.p;
+        a.fade_tab = fade_tab.p;
+        a.fade_lds = fade_lds;
         a.rsum = rsum.p;
         a.render_groups = render_groups;
         a.waves_per_group = cfg.waves_per_group;
@@ -444,7 +449,7 @@ void kst_default_config(kst_config *c) {
     c->form = 1;
     c->compensated = 0;
     c->waves_per_group = 1;
-    c->sort_digit_bits = 8;
+    c->sort_digit_bits = 9;
     c->device = 0;
     c->profile = 0;
 }
@@ -555,6 +560,18 @@ int kst_create(const kst_config *cfg, const int16_t *pool, size_t pool_len,
         b->cand_key.alloc(b->slots) != hipSuccess || b->cand_slot.alloc(b->slots) != hipSuccess ||
         b->st.alloc(1) != hipSuccess)
         return cleanup(fail(KST_EDEVICE, "allocating the select/sort scratch"));
+    // The steal fade's gain for d = 0 .. fade-1 (src/cpu.rs:577-586): the same
+    // correctly rounded f32 division and subtraction the per-frame path does.
+    {
+        const uint32_t fade = c.steal_fade_frames;
+        std::vector<float> tab(fade);
+        for (uint32_t d = 0; d < fade; ++d) tab[d] = 1.0f - (float)d / (float)fade;
+        if (b->fade_tab.alloc(fade) != hipSuccess ||
+            hipMemcpy(b->fade_tab.p, tab.data(), fade * sizeof(float), hipMemcpyHostToDevice) !=
+                hipSuccess)
+            return cleanup(fail(KST_EDEVICE, "uploading the fade table"));
+        b->fade_lds = fade <= FADE_LDS_MAX ? fade : 0;
+    }
     DevState zero{};
     if (hipMemcpy(b->st.p, &zero, sizeof zero, hipMemcpyHostToDevice) != hipSuccess)
         return cleanup(fail(KST_EDEVICE, "initialising the device state"));

@@ -42,8 +42,8 @@ constexpr uint32_t ADMIT_EVEN = 1;
 // MSB-first rounds of SEL_BITS-wide digits. Each round also keeps the smallest
 // and largest key of every bin, so the next digit starts at the highest bit
 // that still varies inside the chosen bin (note-id keys skip their zero high
-// bits). Once the bin holds at most SEL_CAP keys they are ranked exactly in
-// LDS. Per-workgroup histograms are stored without atomics and summed by one
+// bits). Once the bin holds at most SEL_CAP keys, one workgroup finishes the
+// rounds on the collected candidates. Per-workgroup histograms are stored without atomics and summed by one
 // workgroup: the same global address hit from every CU is what costs.
 constexpr uint32_t SEL_BITS = 8;
 constexpr uint32_t SEL_BINS = 1u << SEL_BITS;
@@ -56,12 +56,15 @@ constexpr uint32_t SORT_TPB = 256;
 constexpr uint32_t SORT_IPT = 8;
 constexpr uint32_t SORT_TILE = SORT_TPB * SORT_IPT;
 constexpr uint32_t MAX_SORT_PASSES = 32;
+constexpr uint32_t SORT_MAX_BITS = 9;
+constexpr uint32_t SORT_MAX_RADIX = 1u << SORT_MAX_BITS;
 
 // Wave64 render form: 16-frame sub-tiles, transposed through LDS with a row
 // stride of 68 words (68 = 4 mod 32 keeps the 128-bit column reads
 // bank-conflict free).
 constexpr uint32_t SUBTILE = 16;
 constexpr uint32_t TROW = 68;
+constexpr uint32_t FADE_LDS_MAX = 2048;
 constexpr uint32_t TBUF_FLOATS = 2 * SUBTILE * TROW;  // 32 rows
 // Batch rows summed per workgroup in the first mixdown stage.
 constexpr uint32_t REDUCE_ROWS = 256;
@@ -117,6 +120,8 @@ struct Args {
     BlockResult *results;     // [nblocks]
     // Config.
     uint32_t F, gate_frames, max_voices, max_steal, steal_span, fade;
+    const float *fade_tab;  // [fade]: 1 - d / fade, as the per-frame fade computes it
+    uint32_t fade_lds;      // entries of fade_tab staged in LDS (0: too long to stage)
     uint32_t steal_rule, admit_rule, interp, exp_decay, exp_release;
     float env_floor;
     // Render / reduce.
@@ -129,7 +134,7 @@ struct Args {
     uint32_t *keys[2];
     uint32_t *idx[2];
     uint32_t *sort_hist;  // [passes][radix][tiles_max]
-    uint32_t *sort_tot;   // [passes][radix]
+    uint32_t *sort_tot;   // [passes][radix] digit totals
     uint32_t sort_on, sort_passes, digit_bits, tiles_max;
     uint32_t region_shift, stage_shift, phase_shift, phase_mask, dead_region;
     uint32_t *sel_cnt;             // [SEL_MAX_BLOCKS][SEL_BINS] per-workgroup counts
@@ -156,7 +161,7 @@ void launch_end(const Args &a, hipStream_t s);
 // ---- launchers (kst_render.hip) ----
 // Returns false when the wave64 form does not cover this configuration.
 bool fast_render_supported(uint32_t F, uint32_t gate_frames, uint32_t interp);
-size_t fast_render_lds_bytes(uint32_t waves_per_group);
+size_t fast_render_lds_bytes(uint32_t waves_per_group, uint32_t fade_lds);
 void launch_render_fast(const Args &a, bool chan, hipStream_t s);
 void launch_reduce_fast(const Args &a, bool comp, uint32_t max_rows, hipStream_t s);
 void launch_render_plain(const Args &a, bool chan, hipStream_t s);

@@ -33,9 +33,6 @@ __device__ __forceinline__ uint64_t steal_key(const Args &a, const uint32_t *poo
 __global__ void k_begin(Args a) {
     DevState *st = a.st;
     const uint32_t b = st->block;
-    // Clear the sort totals for this block.
-    const uint32_t radix = 1u << a.digit_bits;
-    for (uint32_t i = threadIdx.x; i < a.sort_passes * radix; i += blockDim.x) a.sort_tot[i] = 0;
     if (threadIdx.x != 0) return;
     // src/gpu/mod.rs:1015-1040 (and src/cpu.rs:330-384, which agrees).
     const uint64_t cap = a.max_voices;
@@ -64,7 +61,7 @@ __global__ void k_begin(Args a) {
     st->sel_top = 64;
     st->sel_k = steal_k > 0 ? (uint32_t)steal_k : 1;
     st->sel_count = (uint32_t)live;
-    st->sel_done = live <= SEL_CAP;  // few enough to rank all at once
+    st->sel_done = live <= SEL_CAP;  // few enough for the final workgroup alone
     st->cand_n = 0;
 }
 
@@ -199,27 +196,41 @@ __global__ void __launch_bounds__(SEL_THREADS) k_sel_hist(Args a) {
     }
 }
 
-// One workgroup sums the per-workgroup statistics and picks the bin.
-__global__ void __launch_bounds__(SEL_BINS) k_sel_refine(Args a, uint32_t blocks) {
+// One workgroup sums the per-workgroup statistics and picks the bin: four
+// threads per bin, each over a quarter of the workgroups' rows.
+__global__ void __launch_bounds__(SEL_THREADS) k_sel_refine(Args a, uint32_t blocks) {
     DevState *st = a.st;
     if (st->steal_k == 0 || st->sel_done) return;
-    __shared__ uint32_t cnt[SEL_BINS], scan[SEL_BINS];
-    __shared__ unsigned long long mn[SEL_BINS], mx[SEL_BINS];
-    const uint32_t t = threadIdx.x;
+    constexpr uint32_t Q = SEL_THREADS / SEL_BINS;
+    __shared__ uint32_t cnt[Q][SEL_BINS], scan[SEL_BINS];
+    __shared__ unsigned long long mn[Q][SEL_BINS], mx[Q][SEL_BINS];
+    const uint32_t t = threadIdx.x, bin = t % SEL_BINS, q = t / SEL_BINS;
     uint32_t c = 0;
     unsigned long long lo = ~0ull, hi = 0;
-    for (uint32_t b = 0; b < blocks; ++b) {
-        const size_t o = (size_t)b * SEL_BINS + t;
+#pragma unroll 8
+    for (uint32_t b = q; b < blocks; b += Q) {
+        const size_t o = (size_t)b * SEL_BINS + bin;
         c += a.sel_cnt[o];
         const unsigned long long x = a.sel_min[o], y = a.sel_max[o];
         lo = x < lo ? x : lo;
         hi = y > hi ? y : hi;
     }
-    cnt[t] = c;
-    mn[t] = lo;
-    mx[t] = hi;
+    cnt[q][bin] = c;
+    mn[q][bin] = lo;
+    mx[q][bin] = hi;
     __syncthreads();
-    sel_choose(st, cnt, mn, mx, scan);
+    if (q == 0) {
+        for (uint32_t j = 1; j < Q; ++j) {
+            c += cnt[j][bin];
+            lo = mn[j][bin] < lo ? mn[j][bin] : lo;
+            hi = mx[j][bin] > hi ? mx[j][bin] : hi;
+        }
+        cnt[0][bin] = c;
+        mn[0][bin] = lo;
+        mx[0][bin] = hi;
+    }
+    __syncthreads();
+    sel_choose(st, cnt[0], mn[0], mx[0], scan);
 }
 
 __device__ __forceinline__ void mark_victim(const Args &a, uint32_t *pool, uint32_t i) {
@@ -263,22 +274,21 @@ __global__ void __launch_bounds__(SEL_THREADS) k_sel_collect(Args a) {
     }
 }
 
-// Finish on the candidates: any rounds still needed (one workgroup, rarely
-// taken), then an exact rank in LDS; the first k are victims.
+// Finish on the candidates with more rounds in one workgroup until the bin
+// holds the k-th key alone; keys below each chosen bin become victims as the
+// candidate list is narrowed, and the last one standing is the k-th.
 __global__ void __launch_bounds__(SEL_THREADS) k_sel_final(Args a) {
     DevState *st = a.st;
     if (st->steal_k == 0) return;
-    __shared__ unsigned long long key[SEL_CAP];
-    __shared__ uint32_t slot[SEL_CAP];
     __shared__ uint32_t cnt[SEL_BINS], scan[SEL_BINS];
     __shared__ unsigned long long mn[SEL_BINS], mx[SEL_BINS];
     __shared__ uint32_t s_n;
     uint32_t *pool = sel2(a.pools, st->parity);
     uint32_t n = st->cand_n;
-    // Each round lowers sel_top by at least SEL_BITS, so 64 / SEL_BITS rounds
-    // always end it; the bound only guards against a broken state.
-    for (uint32_t round = 0; round <= 64 / SEL_BITS && !st->sel_done; ++round) {
-        // Another round over the collected candidates.
+    // Each round lowers sel_top by at least SEL_BITS (the chosen bin's keys
+    // agree on every bit above the digit), and unique keys leave one
+    // candidate when it reaches 0; the bound only guards a broken state.
+    for (uint32_t round = 0; round <= 64 / SEL_BITS && st->sel_count > 1; ++round) {
         const uint32_t top = st->sel_top;
         const uint32_t lo = sel_lo(top);
         const uint32_t dmask = (1u << (top - lo)) - 1;
@@ -324,33 +334,7 @@ __global__ void __launch_bounds__(SEL_THREADS) k_sel_final(Args a) {
         n = s_n;
         __syncthreads();
     }
-    if (n > SEL_CAP) n = SEL_CAP;  // cannot happen: done means n <= SEL_CAP or one key
-    const uint32_t k = st->sel_k;
-    for (uint32_t i = threadIdx.x; i < SEL_CAP; i += blockDim.x) {
-        key[i] = i < n ? a.cand_key[i] : ~0ull;
-        slot[i] = i < n ? a.cand_slot[i] : 0;
-    }
-    __syncthreads();
-    for (uint32_t size = 2; size <= SEL_CAP; size <<= 1) {
-        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-            for (uint32_t i = threadIdx.x; i < SEL_CAP; i += blockDim.x) {
-                const uint32_t j = i ^ stride;
-                if (j > i) {
-                    const bool up = (i & size) == 0;
-                    const unsigned long long x = key[i], y = key[j];
-                    if ((x > y) == up) {
-                        key[i] = y;
-                        key[j] = x;
-                        const uint32_t tt = slot[i];
-                        slot[i] = slot[j];
-                        slot[j] = tt;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    }
-    for (uint32_t i = threadIdx.x; i < k && i < n; i += blockDim.x) mark_victim(a, pool, slot[i]);
+    if (threadIdx.x == 0 && n == 1 && st->sel_k == 1) mark_victim(a, pool, a.cand_slot[0]);
 }
 
 // shaders/spawn.wgsl:35-78; thinning per src/voice.rs:72-75 when the block is
@@ -409,83 +393,84 @@ __device__ __forceinline__ bool key_alive(const Args &a, uint32_t key) {
 }
 
 // Per-tile digit counts of pass p over that pass's input order (the previous
-// pass's output), plus the digit totals. Pass 0 also counts the survivors.
+// pass's output). Each tile's keys are loaded up front; counts go to LDS and
+// then to the tile's column of sort_hist, with no global atomics.
 __global__ void __launch_bounds__(SORT_TPB) k_sort_hist(Args a, uint32_t p) {
     DevState *st = a.st;
     const uint32_t n = st->live_total;
     const uint32_t tiles = (n + SORT_TILE - 1) / SORT_TILE;
     const uint32_t radix = 1u << a.digit_bits, mask = radix - 1;
     const uint32_t shift = p * a.digit_bits;
-    __shared__ uint32_t sh[256];
-    __shared__ uint32_t s_alive;
+    __shared__ uint32_t sh[SORT_MAX_RADIX];
     const uint32_t *keys = sel2(a.keys, p & 1);
     for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
-        for (uint32_t i = threadIdx.x; i < radix; i += SORT_TPB) sh[i] = 0;
-        if (threadIdx.x == 0) s_alive = 0;
-        __syncthreads();
-        uint32_t alive = 0;
+        uint32_t kk[SORT_IPT];
+#pragma unroll
         for (uint32_t k = 0; k < SORT_IPT; ++k) {
-            uint32_t item = t * SORT_TILE + k * SORT_TPB + threadIdx.x;
-            if (item >= n) break;
-            uint32_t key = keys[item];
-            if (p == 0) alive += key_alive(a, key) ? 1u : 0u;
-            atomicAdd(&sh[(key >> shift) & mask], 1u);
+            const uint32_t item = t * SORT_TILE + k * SORT_TPB + threadIdx.x;
+            kk[k] = item < n ? keys[item] : 0u;
         }
-        if (alive) atomicAdd(&s_alive, alive);
+        for (uint32_t i = threadIdx.x; i < radix; i += SORT_TPB) sh[i] = 0;
         __syncthreads();
-        for (uint32_t d = threadIdx.x; d < radix; d += SORT_TPB) {
-            uint32_t c = sh[d];
-            a.sort_hist[(size_t)(p * radix + d) * a.tiles_max + t] = c;
-            if (c) atomicAdd(&a.sort_tot[p * radix + d], c);
+#pragma unroll
+        for (uint32_t k = 0; k < SORT_IPT; ++k) {
+            const uint32_t item = t * SORT_TILE + k * SORT_TPB + threadIdx.x;
+            if (item < n) atomicAdd(&sh[(kk[k] >> shift) & mask], 1u);
         }
-        if (threadIdx.x == 0 && s_alive) atomicAdd(&st->live_new, s_alive);
+        __syncthreads();
+        for (uint32_t d = threadIdx.x; d < radix; d += SORT_TPB)
+            a.sort_hist[(size_t)(p * radix + d) * a.tiles_max + t] = sh[d];
         __syncthreads();
     }
 }
 
-// One workgroup per (pass, digit): the digit's base is the sum of the smaller
-// digits' totals; each tile's offset adds the same digit's counts in the tiles
-// before it.
+// Exclusive scan of `v` (one value per thread of a 256-thread workgroup) in
+// `buf`; returns the exclusive prefix and leaves the total in *total.
+__device__ __forceinline__ uint32_t block_exscan256(uint32_t v, uint32_t *buf, uint32_t *total) {
+    const uint32_t t = threadIdx.x;
+    buf[t] = v;
+    __syncthreads();
+    for (uint32_t off = 1; off < 256; off <<= 1) {
+        const uint32_t x = t >= off ? buf[t - off] : 0u;
+        __syncthreads();
+        buf[t] += x;
+        __syncthreads();
+    }
+    const uint32_t incl = buf[t];
+    *total = buf[255];
+    __syncthreads();
+    return incl - v;
+}
+
+// One workgroup per (pass, digit): each tile's offset within the digit is the
+// same digit's count in the tiles before it; the digit's total goes to
+// sort_tot for the scatter to place the digits.
 __global__ void __launch_bounds__(256) k_sort_scan(Args a, uint32_t p) {
     DevState *st = a.st;
     const uint32_t n = st->live_total;
     const uint32_t tiles = (n + SORT_TILE - 1) / SORT_TILE;
     const uint32_t radix = 1u << a.digit_bits;
     const uint32_t d = blockIdx.x;
-    __shared__ uint32_t red[256];
-    uint32_t s = 0;
-    for (uint32_t i = threadIdx.x; i < d; i += 256) s += a.sort_tot[p * radix + i];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (uint32_t w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-    }
-    uint32_t running = red[0];
-    __syncthreads();
+    __shared__ uint32_t buf[256];
     uint32_t *row = a.sort_hist + (size_t)(p * radix + d) * a.tiles_max;
-    // Exclusive scan of row[0..tiles) in chunks of 256.
+    uint32_t running = 0;
     for (uint32_t base = 0; base < tiles; base += 256) {
-        uint32_t t = base + threadIdx.x;
-        uint32_t v = t < tiles ? row[t] : 0;
-        red[threadIdx.x] = v;
-        __syncthreads();
-        for (uint32_t off = 1; off < 256; off <<= 1) {
-            uint32_t x = threadIdx.x >= off ? red[threadIdx.x - off] : 0;
-            __syncthreads();
-            red[threadIdx.x] += x;
-            __syncthreads();
-        }
-        if (t < tiles) row[t] = running + red[threadIdx.x] - v;
-        uint32_t total = red[255];
-        __syncthreads();
+        const uint32_t t = base + threadIdx.x;
+        const uint32_t v = t < tiles ? row[t] : 0u;
+        uint32_t total;
+        const uint32_t ex = block_exscan256(v, buf, &total);
+        if (t < tiles) row[t] = running + ex;
         running += total;
     }
+    if (threadIdx.x == 0) a.sort_tot[p * radix + d] = running;
 }
 
 // Stable scatter of one digit pass. Items of a tile are ranked in index
 // order: rounds of 256 consecutive items, waves in order within a round,
-// lanes in order within a wave (match masks built from ballots).
+// lanes in order within a wave (match masks built from ballots). A digit's
+// first lane in each wave publishes the wave's count for that digit and
+// clears it again after the round, so the per-wave table stays zero between
+// rounds without a full sweep.
 __global__ void __launch_bounds__(SORT_TPB) k_sort_scatter(Args a, uint32_t p) {
     DevState *st = a.st;
     const uint32_t n = st->live_total;
@@ -496,63 +481,104 @@ __global__ void __launch_bounds__(SORT_TPB) k_sort_scatter(Args a, uint32_t p) {
     const uint32_t *iin = sel2(a.idx, p & 1);
     uint32_t *kout = sel2(a.keys, (p + 1) & 1);
     uint32_t *iout = sel2(a.idx, (p + 1) & 1);
-    __shared__ uint32_t cnt[256];
-    __shared__ uint32_t wc[SORT_TPB / 64][256];
+    constexpr uint32_t WAVES = SORT_TPB / 64;
+    constexpr uint32_t PER = SORT_MAX_RADIX / SORT_TPB;
+    __shared__ uint32_t base[SORT_MAX_RADIX];
+    __shared__ uint32_t cnt[SORT_MAX_RADIX];
+    __shared__ uint32_t wc[WAVES][SORT_MAX_RADIX];
+    __shared__ uint32_t buf[256];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    // Digit bases: exclusive scan of this pass's digit totals.
+    {
+        uint32_t v[PER], sum = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < PER; ++j) {
+            const uint32_t d = threadIdx.x * PER + j;
+            v[j] = d < radix ? a.sort_tot[p * radix + d] : 0u;
+            sum += v[j];
+        }
+        uint32_t total;
+        uint32_t run = block_exscan256(sum, buf, &total);
+#pragma unroll
+        for (uint32_t j = 0; j < PER; ++j) {
+            base[threadIdx.x * PER + j] = run;
+            run += v[j];
+        }
+    }
+    for (uint32_t i = threadIdx.x; i < WAVES * SORT_MAX_RADIX; i += SORT_TPB)
+        wc[i / SORT_MAX_RADIX][i % SORT_MAX_RADIX] = 0;
+    __syncthreads();
     for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
-        for (uint32_t d = threadIdx.x; d < radix; d += SORT_TPB)
-            cnt[d] = a.sort_hist[(size_t)(p * radix + d) * a.tiles_max + t];
+        uint32_t kk[SORT_IPT], ii[SORT_IPT];
+#pragma unroll
         for (uint32_t k = 0; k < SORT_IPT; ++k) {
-            for (uint32_t i = threadIdx.x; i < (SORT_TPB / 64) * radix; i += SORT_TPB)
-                wc[i / radix][i % radix] = 0;
-            __syncthreads();
             const uint32_t item = t * SORT_TILE + k * SORT_TPB + threadIdx.x;
             const bool valid = item < n;
-            uint32_t key = valid ? kin[item] : 0;
-            uint32_t digit = (key >> shift) & mask;
+            kk[k] = valid ? kin[item] : 0u;
+            ii[k] = valid ? (p == 0 ? item : iin[item]) : 0u;
+        }
+        for (uint32_t d = threadIdx.x; d < radix; d += SORT_TPB)
+            cnt[d] = base[d] + a.sort_hist[(size_t)(p * radix + d) * a.tiles_max + t];
+        __syncthreads();
+#pragma unroll
+        for (uint32_t k = 0; k < SORT_IPT; ++k) {
+            const uint32_t item = t * SORT_TILE + k * SORT_TPB + threadIdx.x;
+            const bool valid = item < n;
+            const uint32_t key = kk[k];
+            const uint32_t digit = (key >> shift) & mask;
             uint64_t m = __ballot(valid);
             for (uint32_t bit = 0; bit < a.digit_bits; ++bit) {
-                uint64_t bb = __ballot(valid && ((digit >> bit) & 1));
+                const uint64_t bb = __ballot(valid && ((digit >> bit) & 1));
                 m &= ((digit >> bit) & 1) ? bb : ~bb;
             }
-            uint32_t rank = __popcll(m & lt);
-            if (valid && rank == 0) wc[wave][digit] = __popcll(m);
+            const uint32_t rank = __popcll(m & lt);
+            const bool leader = valid && rank == 0;
+            if (leader) wc[wave][digit] = __popcll(m);
             __syncthreads();
             if (valid) {
                 uint32_t off = cnt[digit] + rank;
                 for (uint32_t w = 0; w < wave; ++w) off += wc[w][digit];
                 kout[off] = key;
-                iout[off] = p == 0 ? item : iin[item];
+                iout[off] = ii[k];
             }
             __syncthreads();
-            for (uint32_t d = threadIdx.x; d < radix; d += SORT_TPB) {
-                uint32_t s = 0;
-                for (uint32_t w = 0; w < SORT_TPB / 64; ++w) s += wc[w][d];
-                cnt[d] += s;
+            if (leader) {
+                atomicAdd(&cnt[digit], (uint32_t)__popcll(m));
+                wc[wave][digit] = 0;
             }
-            __syncthreads();
         }
+        __syncthreads();
     }
 }
 
 // shaders/sort.wgsl:376-400 / compact.wgsl:116-146: survivors move to the other
-// buffer in sorted (or original) order; per-block fields are cleared.
+// buffer in sorted (or original) order; per-block fields are cleared. Dead
+// voices sort after every survivor, so the last survivor's thread records the
+// new live count.
 __global__ void __launch_bounds__(256) k_gather(Args a) {
     DevState *st = a.st;
-    const uint32_t n = st->live_new;
+    const uint32_t n = st->live_total;
     const uint32_t *perm = sel2(a.idx, a.sort_passes & 1);
+    const uint32_t *skey = sel2(a.keys, a.sort_passes & 1);
     const uint32_t *src = sel2(a.pools, st->parity);
     uint32_t *dst = sel2(a.pools, st->parity ^ 1);
     const uint32_t c = a.cap;
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n;
          j += gridDim.x * blockDim.x) {
-        uint32_t s = perm[j];
+        const bool alive = key_alive(a, skey[j]);
+        const bool next_alive = j + 1 < n && key_alive(a, skey[j + 1]);
+        if (alive && !next_alive) st->live_new = j + 1;
+        if (!alive) continue;
+        const uint32_t s = perm[j];
+        uint32_t v[VOICE_FIELDS];
 #pragma unroll
-        for (uint32_t f = 0; f < VOICE_FIELDS; ++f) dst[f * c + j] = src[f * c + s];
-        dst[F_START_REL * c + j] = 0;
-        dst[F_BORN_VARIANT * c + j] = 0;
-        dst[F_STOP_REL * c + j] = 0;
+        for (uint32_t f = 0; f < VOICE_FIELDS; ++f) v[f] = src[f * c + s];
+        v[F_START_REL] = 0;
+        v[F_BORN_VARIANT] = 0;
+        v[F_STOP_REL] = 0;
+#pragma unroll
+        for (uint32_t f = 0; f < VOICE_FIELDS; ++f) dst[f * c + j] = v[f];
     }
 }
 
@@ -577,7 +603,7 @@ void launch_begin(const Args &a, hipStream_t s) { k_begin<<<1, 256, 0, s>>>(a); 
 void launch_select(const Args &a, uint32_t grid, uint32_t rounds, hipStream_t s) {
     for (uint32_t r = 0; r < rounds; ++r) {
         k_sel_hist<<<grid, SEL_THREADS, 0, s>>>(a);
-        k_sel_refine<<<1, SEL_BINS, 0, s>>>(a, grid);
+        k_sel_refine<<<1, SEL_THREADS, 0, s>>>(a, grid);
     }
     k_sel_collect<<<grid, SEL_THREADS, 0, s>>>(a);
     k_sel_final<<<1, SEL_THREADS, 0, s>>>(a);

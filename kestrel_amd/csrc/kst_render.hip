@@ -505,6 +505,140 @@ __device__ __forceinline__ void fast_subtile_dispatch(const Args &a, Voice &v, f
     }
 }
 
+// ---------------------------------------------------------------------------
+// wave64 edge path: a sub-tile in which some lane starts late, fades out after
+// a steal, wraps at its loop end or runs off its sample end. Each of these is a
+// per-lane predicate on the straight-line frame:
+//  * frames before start_rel change nothing and sound nothing;
+//  * the steal fade multiplies by 1 - d / fade from a table computed with the
+//    same division, and the frame at d == fade is silent and kills the voice;
+//  * the phase wraps by one loop span (the step is shorter than the span), and
+//    the tap after the loop end is the loop start, as src/cpu.rs:243-259 has it;
+//  * a non-looping voice dies on the first frame whose index is past its
+//    sample, and its last frame's second tap is the last sample itself.
+// The phase chain and the taps are computed first, as in the straight path;
+// the per-frame loop only needs the phase's low word.
+// ---------------------------------------------------------------------------
+
+template <int INTERP>
+__device__ __forceinline__ bool edge_ok(const Args &a, const Voice &v, uint32_t f0) {
+    if (!v.live || v.stage == ENV_DEAD) return true;
+    const uint32_t hi = (uint32_t)(v.phase >> 32);
+    if ((v.step >> 32) >= (1u << 26) || hi >= 0x80000000u) return false;
+    const bool loop_en = (v.vflags & VF_LOOP) != 0;
+    const bool lur = (v.vflags & VF_LOOP_UNTIL_RELEASE) != 0;
+    const bool looping = loop_en && !(lur && v.stage >= ENV_RELEASE);
+    if (looping && (v.ls >= v.le || hi >= v.le || (uint32_t)(v.step >> 32) >= v.le - v.ls))
+        return false;
+    // Taps reach index le (looping) or len (not looping); all inside the bank.
+    if ((uint64_t)v.base + (looping ? v.le : v.len) >= a.bank_len) return false;
+    // A fade this sub-tile needs the staged table.
+    if (v.stop_rel != 0 && v.stop_rel <= f0 + SUBTILE && a.fade_lds == 0) return false;
+    return true;
+}
+
+template <int INTERP, bool CLAMP>
+__device__ __forceinline__ void fast_subtile_edge(const Args &a, Voice &v, float *tb,
+                                                  const float *ftab, uint32_t lane,
+                                                  uint32_t f0) {
+    const bool sounding = v.live && v.stage != ENV_DEAD;
+    if (!sounding) {
+        v.gl = v.gr = 0.0f;
+        v.step = 0;
+        v.phase = 0;
+        v.base = 0;
+    }
+    const bool loop_en = (v.vflags & VF_LOOP) != 0;
+    const bool lur = (v.vflags & VF_LOOP_UNTIL_RELEASE) != 0;
+    const bool looping = sounding && loop_en && !(lur && v.stage >= ENV_RELEASE);
+    const uint64_t span_hi = (uint64_t)(v.le - v.ls) << 32;
+    const char *bank = reinterpret_cast<const char *>(a.bank);
+    const uint32_t base2 = v.base * 2u;
+    // Frames [f0 + i] that sound: started, not past the sample, not past the fade.
+    uint32_t amask = 0;
+    uint32_t w0[SUBTILE];
+    uint64_t ph = v.phase;
+#pragma unroll
+    for (uint32_t i = 0; i < SUBTILE; ++i) {
+        const uint32_t f = f0 + i;
+        const uint32_t idx = (uint32_t)(ph >> 32);
+        const bool started = f >= v.start_rel;
+        const bool past_end = !looping && idx >= v.len;
+        const bool faded = v.stop_rel != 0 && f + 1 >= v.stop_rel + a.fade;
+        const bool act = sounding && started && !past_end && !faded;
+        uint32_t w = tap_pair(bank, act ? base2 + idx * 2u : 0u);
+        if (INTERP == 1 && act && (looping ? idx + 1 == v.le : idx + 1 == v.len)) {
+            const uint32_t j = looping ? v.ls : idx;
+            w = (w & 0xFFFFu) | ((uint32_t)(uint16_t)a.bank[v.base + j] << 16);
+        }
+        w0[i] = w;
+        amask |= act ? 1u << i : 0u;
+        if (started) {
+            ph += v.step;
+            if (looping && (uint32_t)(ph >> 32) >= v.le) ph -= span_hi;
+        }
+    }
+    float m, c, thi, tlo;
+    env_coefs(a, v, m, c, thi, tlo);
+    const bool filt = (v.p.flags & RP_FILTER) != 0;
+    uint32_t lo = (uint32_t)v.phase;
+    const uint32_t slo = (uint32_t)v.step;
+    bool kill = false;
+    bool died = false;
+#pragma unroll
+    for (uint32_t i = 0; i < SUBTILE; ++i) {
+        const uint32_t f = f0 + i;
+        const bool act = (amask >> i) & 1u;
+        if (kill) {
+            if (v.stage == ENV_DEAD) v.gl = v.gr = 0.0f;
+            kill = false;
+        }
+        const float ln = fmaf(v.level, m, c);
+        const bool tr = act && ((ln >= thi) | (ln <= tlo));
+        v.level = act ? ln : v.level;
+        if (__builtin_expect(__any(tr), 0)) {
+            if (tr) {
+                env_transition(a, v);
+                env_coefs(a, v, m, c, thi, tlo);
+            }
+            kill = __any(tr && v.stage == ENV_DEAD);
+        }
+        const float g = fminf(v.level, 1.0f);
+        const float flo = (float)lo;
+        float s;
+        if (INTERP == 0) {
+            s = norm_lo<CLAMP>(w0[i]);
+        } else {
+            const float s0 = norm_lo<CLAMP>(w0[i]);
+            const float s1 = norm_hi<CLAMP>(w0[i]);
+            s = fmaf((s1 - s0) * flo, FRAC_SCALE, s0);
+        }
+        const float x = s * g;
+        const float b0x = v.p.b0 * x;
+        const float yf = b0x + v.z1;
+        const float z1n = v.p.b1 * x - v.p.a1 * yf + v.z2;
+        const float z2n = b0x - v.p.a2 * yf;
+        const bool upd = act && filt;
+        v.z1 = upd ? z1n : v.z1;
+        v.z2 = upd ? z2n : v.z2;
+        float y = filt ? yf : x;
+        const bool fading = v.stop_rel != 0 && f + 1 >= v.stop_rel;
+        if (fading && act) y = y * ftab[f + 1 - v.stop_rel];
+        tb[(2 * i) * TROW + lane] = act ? y * v.gl : 0.0f;
+        tb[(2 * i + 1) * TROW + lane] = act ? y * v.gr : 0.0f;
+        lo += (f >= v.start_rel) ? slo : 0u;
+        // Past the sample or through the fade: dead from here on.
+        died |= sounding && f >= v.start_rel && !act;
+    }
+    if (kill && v.stage == ENV_DEAD) v.gl = v.gr = 0.0f;
+    v.phase = ph;
+    if (died) {
+        v.stage = ENV_DEAD;
+        v.level = 0.0f;
+        v.gl = v.gr = 0.0f;
+    }
+}
+
 // Error-free transformation of a + b (Knuth's two-sum).
 __device__ __forceinline__ void two_sum(float a, float b, float &s, float &e) {
     s = a + b;
@@ -529,6 +663,9 @@ __global__ void __launch_bounds__(256) k_render_fast(Args a) {
     const uint32_t F = a.F;
     const uint32_t two_f = 2 * F;
     float *tb = lds + wave * TBUF_FLOATS;
+    float *ftab = lds + wpg * TBUF_FLOATS;
+    for (uint32_t i = threadIdx.x; i < a.fade_lds; i += blockDim.x) ftab[i] = a.fade_tab[i];
+    __syncthreads();
 
     DevState *st = a.st;
     const uint32_t live_total = st->live_total;
@@ -551,6 +688,8 @@ __global__ void __launch_bounds__(256) k_render_fast(Args a) {
         if (f0 % a.gate_frames == 0) gate_tile<CHAN>(a, gates, chan, chan_flags, f0 / a.gate_frames, v);
         if (!__any(!fast_ok<INTERP>(a, v, f0))) {
             fast_subtile_dispatch<INTERP, CLAMP>(a, v, tb, lane);
+        } else if (INTERP != 2 && !__any(!edge_ok<INTERP>(a, v, f0))) {
+            fast_subtile_edge<INTERP, CLAMP>(a, v, tb, ftab, lane, f0);
         } else {
 #pragma unroll 1
             for (uint32_t i = 0; i < SUBTILE; ++i) {
@@ -758,7 +897,7 @@ __global__ void __launch_bounds__(1024) k_reduce_plain(Args a) {
 template <int INTERP, bool CHAN, bool CLAMP>
 void launch_fast_t(const Args &a, hipStream_t s) {
     const uint32_t threads = a.waves_per_group * 64;
-    const size_t lds = (size_t)a.waves_per_group * TBUF_FLOATS * sizeof(float);
+    const size_t lds = fast_render_lds_bytes(a.waves_per_group, a.fade_lds);
     k_render_fast<INTERP, CHAN, CLAMP><<<a.render_groups, threads, lds, s>>>(a);
 }
 
@@ -775,8 +914,8 @@ bool fast_render_supported(uint32_t F, uint32_t gate_frames, uint32_t interp) {
     return F % 32 == 0 && gate_frames % SUBTILE == 0 && interp <= 2;
 }
 
-size_t fast_render_lds_bytes(uint32_t waves_per_group) {
-    return (size_t)waves_per_group * TBUF_FLOATS * sizeof(float);
+size_t fast_render_lds_bytes(uint32_t waves_per_group, uint32_t fade_lds) {
+    return ((size_t)waves_per_group * TBUF_FLOATS + ((fade_lds + 3) & ~3u)) * sizeof(float);
 }
 
 void launch_render_fast(const Args &a, bool chan, hipStream_t s) {

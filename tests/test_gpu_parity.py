@@ -210,14 +210,17 @@ def test_validation_errors_name_the_flag():
     g.close()
 
 
-def test_multi_tile_sort_and_steal_at_40k_voices():
+@pytest.mark.parametrize("rule,digit_bits", [(2, 8), (0, 9), (2, 5)])
+def test_multi_tile_sort_and_steal_at_40k_voices(rule, digit_bits):
     """A pool large enough for many sort tiles and select workgroups, saturated
-    so every block steals; the oracle runs in sorted mode."""
-    cfg = W.config_c2(max_voices=40_000)
+    so every block steals; the oracle runs in sorted mode. Oldest keys (note
+    ids) and quietest keys (level over id) take different select rounds; the
+    radix width changes the number of sort passes."""
+    cfg = W.config_c2(max_voices=40_000).replace(steal_rule=rule, sort_digit_bits=digit_bits)
     bank = W.make_bank(11, instruments=16, zones=4, key_lo=0, key_hi=127)
     notes = W.burst_notes(12, 0.8, base_rate_hz=150_000.0, burst_notes_n=20_000,
                           burst_every_s=0.25)
-    print("40k voices", flush=True)
+    print(f"40k voices rule={rule} digit_bits={digit_bits}", flush=True)
     run_stream(cfg, bank, W.BlockBuilder(cfg, bank, notes), 60, check_pool_every=5,
                min_live=39_000)
 
