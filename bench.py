@@ -65,6 +65,19 @@ def render_voices(cfg, live0, blocks, stats):
     return np.array(out, np.float64)
 
 
+def calibrate_copy():
+    import torch
+    n = 1 << 28
+    src = torch.full((n,), 1.0, device="cuda", dtype=torch.float32)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 5}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,11 +91,17 @@ def main():
     ap.add_argument("--seed", type=int, default=0x4B530002)
     ap.add_argument("--waves-per-group", type=int, default=None)
     ap.add_argument("--sort-digit-bits", type=int, default=None)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy 5 times (the reference kernel "
+                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(the render path is single-GPU, see DESIGN.md for how it would shard)")
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the render "
+              "path runs on one GPU (sharding by voice is a design note, not built)")
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
 
     from kestrel_amd import workload as W
     from kestrel_amd.backend import HipSynth

@@ -194,6 +194,7 @@ struct kst_backend {
     // Host side of the per-block API.
     std::vector<uint32_t> h_gates, h_chans;
     uint32_t h_chan_flags = 0;
+    bool staged_chan = true, graph_chan = true;
     std::vector<kst_spawn_cmd> pending;
     uint32_t *pin_small = nullptr;  // pinned staging for per-block uploads
     size_t pin_cmds_cap = 0;
@@ -777,6 +778,8 @@ int kst_stage_blocks(kst_backend *b, uint32_t nblocks, const uint32_t *gates,
                               hipMemcpyHostToDevice, s));
     HIPCHK(hipStreamSynchronize(s));
     b->staged = nblocks;
+    // The controller-row render variant only when some staged block has rows.
+    b->staged_chan = std::any_of(chan_flags, chan_flags + nblocks, [](uint32_t f) { return f != 0; });
     return KST_OK;
 }
 
@@ -790,10 +793,18 @@ int kst_run_staged(kst_backend *b, int use_graph) {
     HIPCHK(hipStreamSynchronize(s));
     // Capacity-sized grids: the live count is on the device only.
     const uint32_t n = b->slots;
+    const bool chan = b->staged_chan;
     if (use_graph) {
+        if (b->graph_exec && b->graph_chan != chan) {
+            (void)hipGraphExecDestroy(b->graph_exec);
+            (void)hipGraphDestroy(b->graph);
+            b->graph_exec = nullptr;
+            b->graph = nullptr;
+        }
         if (!b->graph_exec) {
             HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            b->launch_block(b->cfg.max_voices, n, true, true, true, nullptr);
+            b->launch_block(b->cfg.max_voices, n, true, true, chan, nullptr);
+            b->graph_chan = chan;
             hipError_t e = hipStreamEndCapture(s, &b->graph);
             if (e != hipSuccess) return fail(KST_EDEVICE, "graph capture: %s", hipGetErrorString(e));
             HIPCHK(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
@@ -805,7 +816,7 @@ int kst_run_staged(kst_backend *b, int use_graph) {
             if (rc) return rc;
         }
         for (uint32_t i = 0; i < b->staged; ++i)
-            b->launch_block(b->cfg.max_voices, n, true, true, true,
+            b->launch_block(b->cfg.max_voices, n, true, true, chan,
                             b->profile ? &b->stage_ev[6 * i] : nullptr);
         HIPCHK(hipGetLastError());
     }
