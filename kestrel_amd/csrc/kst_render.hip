@@ -132,20 +132,27 @@ __device__ __forceinline__ uint64_t fixed_scale(uint64_t phase, uint32_t factor)
 }
 
 // Note-off gate and controller row for one gate tile (src/cpu.rs:449-487).
+// Returns true when it changed the stage or the step (what the straight
+// path's phase limit depends on).
 template <bool CHAN>
-__device__ __forceinline__ void gate_tile(const Args &a, const uint32_t *gates,
+__device__ __forceinline__ bool gate_tile(const Args &a, const uint32_t *gates,
                                           const uint32_t *chan, uint32_t chan_flags,
                                           uint32_t gt, Voice &v) {
-    if (!v.live) return;
+    if (!v.live) return false;
+    bool changed = false;
     // The slot is masked to the row (the driver never builds one past 2047).
     if (v.stage < ENV_RELEASE && gates[gt * GATE_SLOTS + (v.gate_slot & (GATE_SLOTS - 1))] >= v.ordinal) {
         v.stage = ENV_RELEASE;
         v.level = fminf(v.level, 1.0f);
+        changed = true;
     }
     if (CHAN && chan_flags) {
         const uint32_t channel = (v.gate_slot >> 7) & 15;
         const uint32_t ci = (gt * BEND_CHANNELS + channel) * CHAN_FIELDS;
-        if (chan_flags & CHAN_ACTIVE_BEND) v.step = fixed_scale(v.base_step, chan[ci + CHAN_BEND]);
+        if (chan_flags & CHAN_ACTIVE_BEND) {
+            v.step = fixed_scale(v.base_step, chan[ci + CHAN_BEND]);
+            changed = true;
+        }
         if (chan_flags & CHAN_ACTIVE_GAIN) {
             v.gl = v.bgl * __uint_as_float(chan[ci + CHAN_GAIN_L]);
             v.gr = v.bgr * __uint_as_float(chan[ci + CHAN_GAIN_R]);
@@ -160,6 +167,7 @@ __device__ __forceinline__ void gate_tile(const Args &a, const uint32_t *gates,
             }
         }
     }
+    return changed;
 }
 
 __device__ __forceinline__ float fetch(const Args &a, uint32_t base, uint32_t idx) {
@@ -355,27 +363,38 @@ __device__ __forceinline__ void env_transition(const Args &a, Voice &v) {
     }
 }
 
-// Can this lane take the straight-line path for frames [f0, f0 + SUBTILE)?
-// Silent lanes (no voice, or DEAD) always can: they are neutralised.
+// The straight path for frames [f0, f0 + SUBTILE) needs every tap of the
+// sub-tile inside the sample (before the loop end when looping) and inside the
+// bank: phase + SUBTILE * step < limit. The limit depends on the stage (loop
+// until release), the step (overflow guard) and fixed voice fields, so it is
+// recomputed only when a gate tile changes one of those.
 template <int INTERP>
-__device__ __forceinline__ bool fast_ok(const Args &a, const Voice &v, uint32_t f0) {
-    if (!v.live || v.stage == ENV_DEAD) return true;
-    if (v.start_rel > f0) return false;
-    if (v.stop_rel != 0 && v.stop_rel <= f0 + SUBTILE) return false;
-    const uint32_t hi = (uint32_t)(v.phase >> 32);
-    // No 64-bit overflow in phase + SUBTILE * step.
-    if ((v.step >> 32) >= (1u << 26) || hi >= 0x80000000u) return false;
-    const uint32_t hi_end = (uint32_t)((v.phase + SUBTILE * v.step) >> 32);
+__device__ __forceinline__ uint64_t fast_limit(const Args &a, const Voice &v) {
+    if ((v.step >> 32) >= (1u << 26)) return 0;  // SUBTILE steps could overflow
     const bool loop_en = (v.vflags & VF_LOOP) != 0;
     const bool lur = (v.vflags & VF_LOOP_UNTIL_RELEASE) != 0;
     const bool looping = loop_en && !(lur && v.stage >= ENV_RELEASE);
-    const uint32_t limit = looping ? v.le : v.len;
-    const uint32_t reach = INTERP == 2 ? 2u : (INTERP == 1 ? 1u : 0u);
-    if ((uint64_t)hi_end + reach >= limit) return false;
-    if (INTERP == 2 && hi < 1) return false;
-    // Every tap stays inside the bank.
-    if ((uint64_t)v.base + hi_end + reach >= a.bank_len) return false;
-    return true;
+    const uint64_t reach = INTERP == 2 ? 2u : (INTERP == 1 ? 1u : 0u);
+    // A branch-free pick: a plain select of the two fields let the compiler
+    // address the voice struct in scratch memory.
+    const uint32_t pick = 0u - (uint32_t)looping;
+    const uint64_t limit = v.len ^ ((v.le ^ v.len) & pick);
+    const uint64_t l1 = limit > reach ? limit - reach : 0;
+    const uint64_t room = (uint64_t)a.bank_len;
+    const uint64_t l2 = room > (uint64_t)v.base + reach ? room - v.base - reach : 0;
+    return (l1 < l2 ? l1 : l2) << 32;  // < 2^63: a phase at or past 2^63 never passes
+}
+
+// Can this lane take the straight-line path for frames [f0, f0 + SUBTILE)?
+// Silent lanes (no voice, or DEAD) always can: they are neutralised.
+template <int INTERP>
+__device__ __forceinline__ bool fast_ok(const Voice &v, uint64_t lim, uint32_t f0) {
+    if (!v.live || v.stage == ENV_DEAD) return true;
+    if (v.start_rel > f0) return false;
+    if (v.stop_rel != 0 && v.stop_rel <= f0 + SUBTILE) return false;
+    if (INTERP == 2 && (v.phase >> 32) == 0) return false;  // tap idx-1
+    // phase + SUBTILE * step < lim, without wrapping (step < 2^58).
+    return v.phase < lim && v.step * SUBTILE < lim - v.phase;
 }
 
 // One 32-bit word holding the int16 frames at byte offsets off and off + 2
@@ -409,22 +428,23 @@ __device__ __forceinline__ void fast_subtile(const Args &a, Voice &v, float *tb,
         v.base = 0;
     }
     const char *bank = reinterpret_cast<const char *>(a.bank);
-    const uint32_t base2 = v.base * 2u;
     // Every frame's taps first: the phase chain is integer arithmetic, so all
     // of the sub-tile's loads can be in flight before the first is needed.
     uint32_t w0[SUBTILE], w1[INTERP == 2 ? SUBTILE : 1];
     {
-        uint64_t ph = v.phase;
+        // base + idx in the high word: one 64-bit add and one shift per frame
+        // (no carry out: the straight path keeps base + idx + reach in the bank).
+        uint64_t at = v.phase + ((uint64_t)v.base << 32);
 #pragma unroll
         for (uint32_t i = 0; i < SUBTILE; ++i) {
-            const uint32_t off = base2 + (uint32_t)(ph >> 32) * 2u;
+            const uint32_t off = (uint32_t)(at >> 32) * 2u;
             if (INTERP == 2) {
                 w0[i] = tap_pair(bank, off - 2);  // frames idx-1, idx
                 w1[i] = tap_pair(bank, off + 2);  // frames idx+1, idx+2
             } else {
                 w0[i] = tap_pair(bank, off);      // frames idx, idx+1
             }
-            ph += v.step;
+            at += v.step;
         }
     }
     float m, c, thi, tlo;
@@ -639,6 +659,17 @@ __device__ __forceinline__ void fast_subtile_edge(const Args &a, Voice &v, float
     }
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup i runs on XCD
+// i % 8). Give each XCD one contiguous run of the sorted pool instead, so its
+// L2 holds the samples of a few regions rather than of all of them. A
+// bijection on [0, n): XCD x takes groups [x*q + min(x, r), ... + q + (x < r)).
+__device__ __forceinline__ uint32_t xcd_group(uint32_t id, uint32_t n) {
+    constexpr uint32_t XCDS = 8;
+    const uint32_t x = id % XCDS, k = id / XCDS;
+    const uint32_t q = n / XCDS, r = n % XCDS;
+    return x * q + (x < r ? x : r) + k;
+}
+
 // Error-free transformation of a + b (Knuth's two-sum).
 __device__ __forceinline__ void two_sum(float a, float b, float &s, float &e) {
     s = a + b;
@@ -669,7 +700,9 @@ __global__ void __launch_bounds__(256) k_render_fast(Args a) {
 
     DevState *st = a.st;
     const uint32_t live_total = st->live_total;
-    const uint32_t batch = blockIdx.x * wpg + wave;
+    const uint32_t groups = (live_total + 64 * wpg - 1) / (64 * wpg);
+    if (blockIdx.x >= groups) return;
+    const uint32_t batch = xcd_group(blockIdx.x, groups) * wpg + wave;
     if (batch * 64 >= live_total) return;
     const uint32_t b = st->block;
     const uint32_t chan_flags = CHAN ? st->chan_flags : 0u;
@@ -683,10 +716,13 @@ __global__ void __launch_bounds__(256) k_render_fast(Args a) {
     const uint32_t vi = batch * 64 + lane;
     Voice v;
     load_voice(a, pool, vi, vi < live_total, v);
+    uint64_t lim = fast_limit<INTERP>(a, v);
     for (uint32_t s = 0; s < F / SUBTILE; ++s) {
         const uint32_t f0 = s * SUBTILE;
-        if (f0 % a.gate_frames == 0) gate_tile<CHAN>(a, gates, chan, chan_flags, f0 / a.gate_frames, v);
-        if (!__any(!fast_ok<INTERP>(a, v, f0))) {
+        if (f0 % a.gate_frames == 0 &&
+            gate_tile<CHAN>(a, gates, chan, chan_flags, f0 / a.gate_frames, v))
+            lim = fast_limit<INTERP>(a, v);
+        if (!__any(!fast_ok<INTERP>(v, lim, f0))) {
             fast_subtile_dispatch<INTERP, CLAMP>(a, v, tb, lane);
         } else if (INTERP != 2 && !__any(!edge_ok<INTERP>(a, v, f0))) {
             fast_subtile_edge<INTERP, CLAMP>(a, v, tb, ftab, lane, f0);
