@@ -395,7 +395,7 @@ struct kst_backend {
         else
             launch_reduce_plain(a, s);
         if (timed) (void)hipEventRecord(ev[4], s);
-        launch_sort(a, sort_grid(n_total), s);
+        launch_sort(a, sort_grid(n_total), per_voice_grid(n_total), s);
         launch_end(a, s);
         if (timed) (void)hipEventRecord(ev[5], s);
     }

@@ -155,7 +155,7 @@ __device__ __forceinline__ T *sel2(T *const (&pair)[2], uint32_t i) {
 void launch_begin(const Args &a, hipStream_t s);
 void launch_select(const Args &a, uint32_t grid, uint32_t rounds, hipStream_t s);
 void launch_spawn(const Args &a, uint32_t grid, hipStream_t s);
-void launch_sort(const Args &a, uint32_t grid, hipStream_t s);
+void launch_sort(const Args &a, uint32_t grid, uint32_t gather_grid, hipStream_t s);
 void launch_end(const Args &a, hipStream_t s);
 
 // ---- launchers (kst_render.hip) ----

@@ -196,41 +196,77 @@ __global__ void __launch_bounds__(SEL_THREADS) k_sel_hist(Args a) {
     }
 }
 
-// One workgroup sums the per-workgroup statistics and picks the bin: four
-// threads per bin, each over a quarter of the workgroups' rows.
+// One workgroup sums the per-workgroup counts (four threads per bin, each over
+// a quarter of the rows) and picks the bin; only that bin's smallest and
+// largest keys are then read back from every row.
 __global__ void __launch_bounds__(SEL_THREADS) k_sel_refine(Args a, uint32_t blocks) {
     DevState *st = a.st;
     if (st->steal_k == 0 || st->sel_done) return;
     constexpr uint32_t Q = SEL_THREADS / SEL_BINS;
+    constexpr uint32_t WAVES = SEL_THREADS / 64;
     __shared__ uint32_t cnt[Q][SEL_BINS], scan[SEL_BINS];
-    __shared__ unsigned long long mn[Q][SEL_BINS], mx[Q][SEL_BINS];
+    __shared__ unsigned long long red_lo[WAVES], red_hi[WAVES];
+    __shared__ uint32_t s_bin, s_excl;
     const uint32_t t = threadIdx.x, bin = t % SEL_BINS, q = t / SEL_BINS;
     uint32_t c = 0;
-    unsigned long long lo = ~0ull, hi = 0;
 #pragma unroll 8
-    for (uint32_t b = q; b < blocks; b += Q) {
-        const size_t o = (size_t)b * SEL_BINS + bin;
-        c += a.sel_cnt[o];
-        const unsigned long long x = a.sel_min[o], y = a.sel_max[o];
+    for (uint32_t b = q; b < blocks; b += Q) c += a.sel_cnt[(size_t)b * SEL_BINS + bin];
+    cnt[q][bin] = c;
+    if (t == 0) s_bin = SEL_BINS;
+    __syncthreads();
+    if (q == 0) {
+        for (uint32_t j = 1; j < Q; ++j) c += cnt[j][bin];
+        cnt[0][bin] = c;
+        scan[bin] = c;
+    }
+    __syncthreads();
+    for (uint32_t off = 1; off < SEL_BINS; off <<= 1) {
+        const uint32_t x = (t < SEL_BINS && t >= off) ? scan[t - off] : 0u;
+        __syncthreads();
+        if (t < SEL_BINS) scan[t] += x;
+        __syncthreads();
+    }
+    const uint32_t k = st->sel_k;
+    if (t < SEL_BINS) {
+        const uint32_t excl = scan[t] - cnt[0][t];
+        if (excl < k && excl + cnt[0][t] >= k) {
+            s_bin = t;
+            s_excl = excl;
+        }
+    }
+    __syncthreads();
+    const uint32_t chosen = s_bin;
+    if (chosen >= SEL_BINS) return;  // k outside the counts: cannot happen
+    unsigned long long lo = ~0ull, hi = 0;
+    for (uint32_t b = t; b < blocks; b += SEL_THREADS) {
+        const unsigned long long x = a.sel_min[(size_t)b * SEL_BINS + chosen];
+        const unsigned long long y = a.sel_max[(size_t)b * SEL_BINS + chosen];
         lo = x < lo ? x : lo;
         hi = y > hi ? y : hi;
     }
-    cnt[q][bin] = c;
-    mn[q][bin] = lo;
-    mx[q][bin] = hi;
-    __syncthreads();
-    if (q == 0) {
-        for (uint32_t j = 1; j < Q; ++j) {
-            c += cnt[j][bin];
-            lo = mn[j][bin] < lo ? mn[j][bin] : lo;
-            hi = mx[j][bin] > hi ? mx[j][bin] : hi;
-        }
-        cnt[0][bin] = c;
-        mn[0][bin] = lo;
-        mx[0][bin] = hi;
+    lo = wave_min_u64(lo);
+    hi = wave_max_u64(hi);
+    if ((t & 63) == 0) {
+        red_lo[t >> 6] = lo;
+        red_hi[t >> 6] = hi;
     }
     __syncthreads();
-    sel_choose(st, cnt[0], mn[0], mx[0], scan);
+    if (t == 0) {
+        for (uint32_t w = 1; w < WAVES; ++w) {
+            lo = red_lo[w] < lo ? red_lo[w] : lo;
+            hi = red_hi[w] > hi ? red_hi[w] : hi;
+        }
+        // The bin's keys lie in [lo, hi]: they share every bit above the
+        // highest one in which those two differ.
+        const unsigned long long diff = lo ^ hi;
+        const uint32_t top = diff ? 64 - __clzll(diff) : 0;
+        const uint32_t n = cnt[0][chosen];
+        st->sel_prefix = top >= 64 ? 0ull : (lo >> top) << top;
+        st->sel_top = top;
+        st->sel_k = k - s_excl;
+        st->sel_count = n;
+        st->sel_done = n <= SEL_CAP || top == 0;
+    }
 }
 
 __device__ __forceinline__ void mark_victim(const Args &a, uint32_t *pool, uint32_t i) {
@@ -613,14 +649,14 @@ void launch_spawn(const Args &a, uint32_t grid, hipStream_t s) {
     k_spawn<<<grid, 256, 0, s>>>(a);
 }
 
-void launch_sort(const Args &a, uint32_t grid, hipStream_t s) {
+void launch_sort(const Args &a, uint32_t grid, uint32_t gather_grid, hipStream_t s) {
     const uint32_t radix = 1u << a.digit_bits;
     for (uint32_t p = 0; p < a.sort_passes; ++p) {
         k_sort_hist<<<grid, SORT_TPB, 0, s>>>(a, p);
         k_sort_scan<<<radix, 256, 0, s>>>(a, p);
         k_sort_scatter<<<grid, SORT_TPB, 0, s>>>(a, p);
     }
-    k_gather<<<grid, 256, 0, s>>>(a);
+    k_gather<<<gather_grid, 256, 0, s>>>(a);
 }
 
 void launch_end(const Args &a, hipStream_t s) { k_end<<<1, 1, 0, s>>>(a); }

@@ -776,15 +776,27 @@ __global__ void __launch_bounds__(256) k_reduce_rows(Args a) {
     __shared__ float sh[2][4][64];
     float hi = 0.0f, lo = 0.0f;
     if (col < two_f) {
-        for (uint32_t r = r0 + ph; r < r1; r += 4) {
-            const float x = a.partials[(size_t)r * two_f + col];
-            if (COMP) {
-                float s, e;
-                two_sum(hi, x, s, e);
-                hi = s;
-                lo = lo + e;
-            } else {
-                hi = hi + x;
+        // Rows r0+ph, r0+ph+4, ... in order; loads issued RU at a time ahead
+        // of the dependent sum.
+        constexpr uint32_t RU = 8;
+        for (uint32_t r = r0 + ph; r < r1; r += 4 * RU) {
+            float x[RU];
+#pragma unroll
+            for (uint32_t u = 0; u < RU; ++u) {
+                const uint32_t rr = r + 4 * u;
+                x[u] = rr < r1 ? a.partials[(size_t)rr * two_f + col] : 0.0f;
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < RU; ++u) {
+                if (r + 4 * u >= r1) break;
+                if (COMP) {
+                    float s, e;
+                    two_sum(hi, x[u], s, e);
+                    hi = s;
+                    lo = lo + e;
+                } else {
+                    hi = hi + x[u];
+                }
             }
         }
     }
@@ -819,15 +831,26 @@ __global__ void __launch_bounds__(256) k_reduce_final(Args a) {
     const uint32_t rows = (st->live_total + 63) / 64;
     const uint32_t chunks = (rows + REDUCE_ROWS - 1) / REDUCE_ROWS;
     float h = 0.0f, l = 0.0f;
-    for (uint32_t c = 0; c < chunks; ++c) {
-        const float x = a.rsum[((size_t)c * two_f + col) * 2];
-        if (COMP) {
-            float s, e;
-            two_sum(h, x, s, e);
-            h = s;
-            l = l + (e + a.rsum[((size_t)c * two_f + col) * 2 + 1]);
-        } else {
-            h = h + x;
+    constexpr uint32_t CU = 8;
+    for (uint32_t c0 = 0; c0 < chunks; c0 += CU) {
+        float2 x[CU];
+#pragma unroll
+        for (uint32_t u = 0; u < CU; ++u) {
+            const uint32_t c = c0 + u;
+            x[u] = c < chunks ? reinterpret_cast<const float2 *>(a.rsum)[(size_t)c * two_f + col]
+                              : make_float2(0.0f, 0.0f);
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < CU; ++u) {
+            if (c0 + u >= chunks) break;
+            if (COMP) {
+                float s, e;
+                two_sum(h, x[u].x, s, e);
+                h = s;
+                l = l + (e + x[u].y);
+            } else {
+                h = h + x[u].x;
+            }
         }
     }
     const float v = COMP ? h + l : h;
