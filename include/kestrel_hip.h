@@ -161,7 +161,32 @@ int kst_run_staged(kst_backend *b, int use_graph);
  * per-block stats (nblocks entries, as stats() would read after each render). */
 int kst_staged_results(kst_backend *b, float *out, kst_block_stats *stats);
 
+/* ---- precision form: FP64 coefficient design (no counterpart in the reference) ----
+ *
+ * The reference designs each RegionParams entry on the host in f32 (make_params
+ * and biquad_lowpass, src/bank.rs:876-979). This designs a whole params variant
+ * on the device in FP64 from the region's envelope times, cutoff and resonance,
+ * with the reference's formulas (key scaling and controller modifiers neutral),
+ * and rounds each coefficient once to f32. Uses the backend's sample_rate,
+ * decay_curve, release_curve and env_floor. */
+typedef struct kst_region_design {
+    double attack_s, hold_s, decay_s;  /* seconds */
+    double sustain;                    /* level, clamped to [0, 1] */
+    double release_s;                  /* seconds */
+    double fc_cents;                   /* lowpass cutoff, absolute cents (8.176 Hz * 2^(c/1200)) */
+    double q_cb;                       /* resonance, centibels */
+    uint32_t filter;                   /* 0: this region has no filter */
+    uint32_t pad;
+} kst_region_design;
+
+/* Design params variant `index` (n == params per variant). Same validation as
+ * kst_set_params_variant. */
+int kst_design_params(kst_backend *b, uint32_t index, const kst_region_design *in, size_t n);
+
 /* ---- test and checkpoint hooks (no counterpart in the reference) ---- */
+
+/* Copy params variant `index` back to the host (n == params per variant). */
+int kst_params_export(kst_backend *b, uint32_t index, kst_region_params *out, size_t n);
 
 /* The live pool in the 24-field SoA layout of shaders/common.wgsl:15-44:
  * field f of voice i at soa[f * stride + i]. Returns the live count. */

@@ -36,6 +36,14 @@ PARAMS_DTYPE = np.dtype([
 ])
 assert PARAMS_DTYPE.itemsize == 48
 
+# kst_region_design: FP64 inputs of the device coefficient design (64 B)
+DESIGN_DTYPE = np.dtype([
+    ("attack_s", "<f8"), ("hold_s", "<f8"), ("decay_s", "<f8"), ("sustain", "<f8"),
+    ("release_s", "<f8"), ("fc_cents", "<f8"), ("q_cb", "<f8"), ("filter", "<u4"),
+    ("pad", "<u4"),
+])
+assert DESIGN_DTYPE.itemsize == 64
+
 VOICE_FIELDS = 24
 # shaders/common.wgsl:15-43
 FIELDS = ("phase_lo", "phase_hi", "step_lo", "step_hi", "smp_base", "smp_len", "loop_start",
@@ -72,6 +80,8 @@ SIGNATURES = {
     "kst_stage_blocks": (i32, [P, u32, P, P, P, P, P]),
     "kst_run_staged": (i32, [P, i32]),
     "kst_staged_results": (i32, [P, P, P]),
+    "kst_design_params": (i32, [P, u32, P, sz]),
+    "kst_params_export": (i32, [P, u32, P, sz]),
     "kst_pool_export": (i64, [P, P, sz]),
     "kst_pool_import": (i32, [P, P, sz, sz]),
     "kst_set_counters": (i32, [P, u64, u64]),

@@ -34,6 +34,7 @@ class HipSynth:
         check(lib.kst_create(C.byref(self._k), ptr(pool), pool.size, ptr(params), params.size,
                              int(n_regions), int(max_sample_len), C.byref(h)))
         self._h = h
+        self._n_params = max(int(params.size), 1)  # an empty table gets one neutral entry
         self.block_samples = cfg.block_frames * 2
         self.gate_words = cfg.tiles() * 2048
         self.chan_words = cfg.tiles() * 64
@@ -68,6 +69,16 @@ class HipSynth:
     def set_params_variant(self, index: int, data: np.ndarray) -> None:
         d = np.ascontiguousarray(data, dtype=PARAMS_DTYPE)
         check(self._lib.kst_set_params_variant(self._h, int(index), ptr(d), d.size))
+
+    def design_params(self, index: int, designs: np.ndarray) -> None:
+        """Design params variant `index` on the device in FP64 (precision form)."""
+        d = np.ascontiguousarray(designs, dtype=abi.DESIGN_DTYPE)
+        check(self._lib.kst_design_params(self._h, int(index), ptr(d), d.size))
+
+    def params_export(self, index: int) -> np.ndarray:
+        out = np.zeros(self._n_params, abi.PARAMS_DTYPE)
+        check(self._lib.kst_params_export(self._h, int(index), ptr(out), out.size))
+        return out
 
     def spawn(self, cmds: np.ndarray) -> None:
         c = np.ascontiguousarray(cmds, dtype=SPAWN_DTYPE)

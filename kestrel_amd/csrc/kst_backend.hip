@@ -632,6 +632,41 @@ int kst_set_params_variant(kst_backend *b, uint32_t index, const kst_region_para
     return KST_OK;
 }
 
+int kst_design_params(kst_backend *b, uint32_t index, const kst_region_design *in, size_t n) {
+    if (!b || (!in && n)) return fail(KST_EINVAL, "null argument");
+    if (n != b->ppv)
+        return fail(KST_EINVAL, "params variant %u has %zu entries, expected %u", index, n, b->ppv);
+    if (index >= b->cfg.max_param_variants)
+        return fail(KST_EINVAL, "params variant %u is past the configured maximum", index);
+    HIPCHK(hipSetDevice(b->device));
+    kst_region_design *d = nullptr;
+    HIPCHK(hipMalloc(&d, n * sizeof(kst_region_design)));
+    hipError_t e = hipMemcpy(d, in, n * sizeof(kst_region_design), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        launch_design(d, (uint32_t)n, b->params.p + (size_t)index * b->ppv,
+                      (double)b->cfg.sample_rate, b->cfg.decay_curve == 0,
+                      b->cfg.release_curve == 0, b->cfg.env_floor, b->stream);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(KST_EDEVICE, "designing params: %s", hipGetErrorString(e));
+    return KST_OK;
+}
+
+int kst_params_export(kst_backend *b, uint32_t index, kst_region_params *out, size_t n) {
+    if (!b || (!out && n)) return fail(KST_EINVAL, "null argument");
+    if (n != b->ppv)
+        return fail(KST_EINVAL, "params variant %u has %zu entries, expected %u", index, n, b->ppv);
+    if (index >= b->cfg.max_param_variants)
+        return fail(KST_EINVAL, "params variant %u is past the configured maximum", index);
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    HIPCHK(hipMemcpy(out, b->params.p + (size_t)index * b->ppv, n * sizeof(kst_region_params),
+                     hipMemcpyDeviceToHost));
+    return KST_OK;
+}
+
 int kst_spawn(kst_backend *b, const kst_spawn_cmd *cmds, size_t n) {
     if (!b || (!cmds && n)) return fail(KST_EINVAL, "null argument");
     if (n > 0xFFFFFFFFull) return fail(KST_EINVAL, "%zu spawn commands in one block", n);

@@ -163,6 +163,8 @@ void launch_end(const Args &a, hipStream_t s);
 bool fast_render_supported(uint32_t F, uint32_t gate_frames, uint32_t interp);
 size_t fast_render_lds_bytes(uint32_t waves_per_group, uint32_t fade_lds);
 void launch_render_fast(const Args &a, bool chan, hipStream_t s);
+void launch_design(const kst_region_design *in, uint32_t n, kst_region_params *out, double sr,
+                   bool exp_decay, bool exp_release, float env_floor, hipStream_t s);
 void launch_reduce_fast(const Args &a, bool comp, uint32_t max_rows, hipStream_t s);
 void launch_render_plain(const Args &a, bool chan, hipStream_t s);
 void launch_reduce_plain(const Args &a, hipStream_t s);

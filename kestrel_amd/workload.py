@@ -127,8 +127,8 @@ def velocity_gain(vel: np.ndarray) -> np.ndarray:
 @dataclasses.dataclass
 class Bank:
     pool: np.ndarray          # int16, every sample followed by POOL_GUARD zeros
-    params: np.ndarray        # PARAMS_DTYPE, one entry per region
-    smp_start: np.ndarray     # per region
+    params: np.ndarray        # PARAMS_DTYPE, one entry per region (f32 host design)
+    smp_start: np.ndarray     # per region (the left channel of a stereo region)
     smp_len: np.ndarray
     loop_start: np.ndarray
     loop_end: np.ndarray
@@ -138,6 +138,23 @@ class Bank:
     key_hi: np.ndarray
     instruments: int
     zones: int
+    design: np.ndarray = None     # DESIGN_DTYPE per region: the inputs of `params`
+    smp_start_r: np.ndarray = None  # right channel of stereo regions, else None
+
+    @property
+    def stereo(self) -> bool:
+        return self.smp_start_r is not None
+
+    def params_f64(self, cfg: Config) -> np.ndarray:
+        """The same design carried out in f64 by the oracle (precision form)."""
+        from oracle import oracle
+        out = np.zeros(len(self.design), oracle.PARAMS64_DTYPE)
+        for i, d in enumerate(self.design):
+            out[i] = oracle.make_params_f64(
+                d["attack_s"], d["hold_s"], d["decay_s"], d["sustain"], d["release_s"],
+                d["fc_cents"], d["q_cb"], float(cfg.sample_rate), cfg.decay_curve == 0,
+                cfg.release_curve == 0, float(F32(cfg.env_floor)), bool(d["filter"]))
+        return out
 
     @property
     def n_regions(self) -> int:
@@ -183,37 +200,40 @@ def make_bank(seed: int, instruments: int = 16, zones: int = 4, key_lo: int = 21
               loop_mode: int = VF_LOOP | VF_LOOP_UNTIL_RELEASE, rate: int = 48000,
               adsr=(0.005, 0.100, 0.6, 0.150), cutoff_hz=(2000.0, 12000.0),
               q=0.7071067811865476, wave: str = "sine",
-              filter_enabled: bool = True) -> Bank:
-    """`instruments` x `zones` regions over [key_lo, key_hi], one mono int16
-    sample per region at its zone's root pitch, seeded."""
+              filter_enabled: bool = True, stereo: bool = False) -> Bank:
+    """`instruments` x `zones` regions over [key_lo, key_hi], one int16 sample
+    per region (two, left and right, when `stereo`) at its zone's root pitch,
+    seeded. `q` is a fixed resonance or a (lo, hi) range drawn log-uniform."""
+    from .abi import DESIGN_DTYPE
     rng = np.random.default_rng(seed)
     nkeys = key_hi - key_lo + 1
     span = -(-nkeys // zones)
-    chunks, starts, lens, roots, klo, khi, params = [], [], [], [], [], [], []
+    chunks, starts, starts_r, lens, roots, klo, khi, params = [], [], [], [], [], [], [], []
+    designs = []
     off = 0
     a, d, s, r = adsr
+    gen = saw_noise_sample if wave == "saw" else sine_noise_sample
     for inst in range(instruments):
         for z in range(zones):
             lo = key_lo + z * span
             hi = min(lo + span - 1, key_hi)
             root = (lo + hi) // 2
             freq = 440.0 * 2 ** ((root - 69) / 12)
-            if wave == "saw":
-                smp = saw_noise_sample(rng, frames, freq, rate)
-            else:
-                smp = sine_noise_sample(rng, frames, freq, rate)
-            chunks.append(smp)
-            chunks.append(np.zeros(POOL_GUARD, np.int16))
-            starts.append(off)
+            for ch in range(2 if stereo else 1):
+                chunks.append(gen(rng, frames, freq, rate))
+                chunks.append(np.zeros(POOL_GUARD, np.int16))
+                (starts_r if ch else starts).append(off)
+                off += frames + POOL_GUARD
             lens.append(frames)
-            off += frames + POOL_GUARD
             roots.append(root)
             klo.append(lo)
             khi.append(hi)
             fc = rng.uniform(*cutoff_hz)
             qq = q if np.isscalar(q) else math.exp(rng.uniform(math.log(q[0]), math.log(q[1])))
-            params.append(make_params(a, 0.0, d, s, r, hz_to_cents(fc), q_to_cb(qq), rate,
+            fc_cents, q_cb = hz_to_cents(fc), q_to_cb(qq)
+            params.append(make_params(a, 0.0, d, s, r, fc_cents, q_cb, rate,
                                       filter_enabled=filter_enabled))
+            designs.append((a, 0.0, d, s, r, fc_cents, q_cb, int(filter_enabled), 0))
     n = len(starts)
     flags = loop_mode if loop[1] > loop[0] + 1 else 0
     return Bank(
@@ -223,7 +243,8 @@ def make_bank(seed: int, instruments: int = 16, zones: int = 4, key_lo: int = 21
         loop_start=np.full(n, loop[0], np.uint32), loop_end=np.full(n, loop[1], np.uint32),
         loop_flags=np.full(n, flags, np.uint32), root_key=np.array(roots, np.int64),
         key_lo=np.array(klo, np.int64), key_hi=np.array(khi, np.int64),
-        instruments=instruments, zones=zones)
+        instruments=instruments, zones=zones, design=np.array(designs, dtype=DESIGN_DTYPE),
+        smp_start_r=np.array(starts_r, np.uint32) if stereo else None)
 
 
 # ---------------------------------------------------------------------------
@@ -583,6 +604,20 @@ class BlockBuilder:
             cmds["gain_l"] = gain * F32(np.cos(theta))
             cmds["gain_r"] = gain * F32(np.sin(theta))
             self.next_id += m
+            if b.stereo:
+                # A stereo sample is two hard-panned mono voices
+                # (src/sfz.rs:10-11, 364-368): left then right, each its own id.
+                two = np.repeat(cmds, 2)
+                two["smp_base"][1::2] = b.smp_start_r[region]
+                two["gain_l"][0::2] = gain
+                two["gain_r"][0::2] = 0.0
+                two["gain_l"][1::2] = 0.0
+                two["gain_r"][1::2] = gain
+                ids2 = np.arange(self.next_id - m, self.next_id - m + 2 * m, dtype=np.uint64)
+                two["note_id_lo"] = (ids2 & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+                two["note_id_hi"] = (ids2 >> np.uint64(32)).astype(np.uint32)
+                self.next_id += m
+                cmds = two
 
         w = BEND_CHANNELS * CHAN_FIELDS
         chan = np.zeros((self.tiles, BEND_CHANNELS, CHAN_FIELDS), np.uint32)
@@ -633,6 +668,17 @@ def config_c2(**kw) -> Config:
     return Config(**base)
 
 
+def config_c3(**kw) -> Config:
+    """Precision form: F=512, 2,097,152 voices, cubic, compensated mixdown.
+    Steal rule Oldest, so f32 and f64 renders of the same stream keep the same
+    voices (Quietest ranks on the envelope level, which the precisions round
+    differently)."""
+    base = dict(block_frames=512, max_voices=1 << 21, interpolation=2, max_steal_percent=10,
+                steal_rule=0, compensated=True)
+    base.update(kw)
+    return Config(**base)
+
+
 def workload_c1(seed: int = 0x4B530001, cfg: Config | None = None):
     cfg = cfg or config_c1()
     bank = make_bank(seed, instruments=16, zones=4, key_lo=21, key_hi=108)
@@ -644,5 +690,17 @@ def workload_c2(seed: int = 0x4B530002, seconds: float = 6.0, cfg: Config | None
                 **note_kw):
     cfg = cfg or config_c2()
     bank = make_bank(seed, instruments=16, zones=4, key_lo=0, key_hi=127)
+    notes = burst_notes(seed + 1, seconds, cfg.sample_rate, **note_kw)
+    return cfg, bank, notes
+
+
+def workload_c3(seed: int = 0x4B530003, seconds: float = 6.0, cfg: Config | None = None,
+                **note_kw):
+    """Stereo band-limited saw + noise samples of 65536 frames (each note two
+    hard-panned voices), resonant lowpass with Q log-uniform in 0.5-20, keys
+    0-127, burst note-ons; bends come from `random_walk_bends` (+-2 semitones)."""
+    cfg = cfg or config_c3()
+    bank = make_bank(seed, instruments=16, zones=4, key_lo=0, key_hi=127, frames=65536,
+                     loop=(16384, 60000), wave="saw", q=(0.5, 20.0), stereo=True)
     notes = burst_notes(seed + 1, seconds, cfg.sample_rate, **note_kw)
     return cfg, bank, notes

@@ -1,0 +1,140 @@
+"""Precision form on the device: FP64 coefficient design, and the error of the
+f32 render against an FP64 render of the same algorithm.
+
+Bars:
+* the device's FP64 design, rounded to f32, equals the oracle's FP64 design
+  (glibc pow/sin/cos) rounded to f32: every coefficient within one f32 ulp,
+  and all but a handful of them identical;
+* a stream rendered with the device-designed table stays bit-exact against
+  the oracle given the same table (pool state) and nulls to NULL_DB;
+* against the oracle's FP64 render (f64 design, f64 voice arithmetic, f64
+  mix) the device's compensated f32 mix is no worse than the oracle's own f32
+  render of the same stream plus a 1 dB margin.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from kestrel_amd import workload as W
+from kestrel_amd.abi import DESIGN_DTYPE
+
+pytestmark = pytest.mark.gpu
+
+NULL_DB = -100.0
+
+
+def db(x):
+    return -math.inf if x == 0 else 20.0 * math.log10(x)
+
+
+def ulp_diff(a, b):
+    a = np.asarray(a, np.float32).view(np.int32).astype(np.int64)
+    b = np.asarray(b, np.float32).view(np.int32).astype(np.int64)
+    return np.abs(a - b)
+
+
+def design_grid():
+    rows = []
+    for q in (0.5, 0.7071067811865476, 2.0, 8.0, 20.0, 60.0):
+        for fc_hz in (25.0, 80.0, 440.0, 3000.0, 12000.0, 23000.0):
+            for t in ((0.0, 0.0, 0.001, 0.0, 0.0005), (0.005, 0.01, 0.1, 0.6, 0.15),
+                      (0.5, 0.2, 3.0, 0.0, 8.0)):
+                a, h, d, s, r = t
+                rows.append((a, h, d, s, r, W.hz_to_cents(fc_hz), W.q_to_cb(q), 1, 0))
+    rows.append((0.01, 0.0, 0.2, 0.5, 0.3, 14000.0, 0.0, 1, 0))  # "no filter asked"
+    rows.append((0.01, 0.0, 0.2, 0.5, 0.3, W.hz_to_cents(1000.0), 0.0, 0, 0))  # filter off
+    return np.array(rows, dtype=DESIGN_DTYPE)
+
+
+def test_device_fp64_design_matches_the_fp64_oracle():
+    from kestrel_amd.backend import HipSynth
+    from oracle import oracle
+    cfg = W.config_c1()
+    d = design_grid()
+    bank = W.make_bank(1, instruments=len(d), zones=1, frames=256, loop=(0, 0))
+    g = HipSynth(cfg, bank.pool, bank.params, bank.n_regions, bank.max_sample_len)
+    g.design_params(1, d)
+    got = g.params_export(1)
+    assert np.array_equal(g.params_export(0), bank.params)  # variant 0 untouched
+    g.close()
+    fields = ("attack_rate", "attack_end", "decay_coef", "decay_target", "sustain",
+              "release_coef", "b0", "b1", "a1", "a2")
+    worst, same, total, f32_worst = 0, 0, 0, 0.0
+    for i, row in enumerate(d):
+        want = oracle.make_params_f64(row["attack_s"], row["hold_s"], row["decay_s"],
+                                      row["sustain"], row["release_s"], row["fc_cents"],
+                                      row["q_cb"], float(cfg.sample_rate), True, True,
+                                      float(np.float32(cfg.env_floor)), bool(row["filter"]))
+        ref32 = oracle.make_params(row["attack_s"], row["hold_s"], row["decay_s"],
+                                   row["sustain"], row["release_s"], row["fc_cents"],
+                                   row["q_cb"], float(cfg.sample_rate), True, True,
+                                   float(np.float32(cfg.env_floor)), bool(row["filter"]))
+        assert got[i]["flags"] == want["flags"], i
+        for f in fields:
+            u = int(ulp_diff(got[i][f], np.float32(want[f])))
+            worst = max(worst, u)
+            same += u == 0
+            total += 1
+            if want["flags"] and f in ("b0", "b1") and want[f] != 0:
+                f32_worst = max(f32_worst, abs(float(ref32[f]) - want[f]) / abs(want[f]))
+    print(f"device FP64 design: {same}/{total} coefficients identical to the oracle's, "
+          f"worst {worst} ulp; the f32 host design's worst relative error in b0/b1 "
+          f"{f32_worst:.2e}")
+    assert worst <= 1
+    assert total - same <= total // 100
+
+
+def c3_small(max_voices=30_000, seconds=1.0, rate=60_000.0):
+    cfg = W.config_c3(max_voices=max_voices)
+    _, bank, notes = W.workload_c3(seconds=seconds, cfg=cfg, base_rate_hz=rate,
+                                   burst_notes_n=5_000, burst_every_s=0.25)
+    return cfg, bank, notes
+
+
+def test_device_designed_stream_is_bit_exact_and_tracks_fp64():
+    """Cubic, stereo, bends, resonant filters, params designed on the device."""
+    from kestrel_amd.backend import HipSynth
+    from oracle import oracle
+    cfg, bank, notes = c3_small()
+    g = HipSynth(cfg, bank.pool, bank.params, bank.n_regions, bank.max_sample_len)
+    g.design_params(0, bank.design)
+    table = g.params_export(0)
+    o32 = oracle.Oracle(cfg, bank.pool, table)
+    o32.set_sorted(bank.n_regions, bank.max_sample_len, True)
+    o64 = oracle.Oracle(cfg, bank.pool, table, precision=1)
+    o64.set_params64(bank.params_f64(cfg))
+    o64.set_sorted(bank.n_regions, bank.max_sample_len, True)
+    bb = W.BlockBuilder(cfg, bank, notes, bends=W.random_walk_bends(17, 2.0, 0.05))
+    err_g = err_c = peak = 0.0
+    sq_g = sq_c = n = 0.0
+    worst_null = -math.inf
+    live = 0
+    for b in range(40):
+        blk = bb.next_block()
+        for be in (g, o32, o64):
+            be.set_gates(blk.gates)
+            be.set_channels(blk.chans, *blk.flags)
+            be.spawn(blk.cmds)
+        og = g.render().astype(np.float64)
+        oc = o32.render_f64()
+        o8 = o64.render_f64()
+        pg, pc = g.pool_export(), o32.pool_export()
+        assert pg.shape == pc.shape and np.array_equal(pg, pc), f"block {b}"
+        live = max(live, g.stats()["active_voices"])
+        peak = max(peak, float(np.abs(o8).max()))
+        worst_null = max(worst_null, float(np.abs(og - oc).max()))
+        err_g = max(err_g, float(np.abs(og - o8).max()))
+        err_c = max(err_c, float(np.abs(oc - o8).max()))
+        sq_g += float(((og - o8) ** 2).sum())
+        sq_c += float(((oc - o8) ** 2).sum())
+        n += og.size
+    g.close()
+    assert live > 20_000
+    rel = lambda x: db(x / peak)
+    print(f"vs FP64 render: device compensated f32 peak {rel(err_g):.1f} dB, rms "
+          f"{rel(math.sqrt(sq_g / n)):.1f} dB; CPU f32 render peak {rel(err_c):.1f} dB, rms "
+          f"{rel(math.sqrt(sq_c / n)):.1f} dB (re peak {peak:.3f}); device vs CPU f32 null "
+          f"{rel(worst_null):.1f} dB")
+    assert rel(worst_null) <= NULL_DB
+    assert rel(err_g) <= rel(err_c) + 1.0
