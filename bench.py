@@ -1,10 +1,14 @@
 #!/usr/bin/env python3
 """Headline benchmark: voice-samples/s and x-realtime at ~1M voices.
 
-Workload: the plain method at saturation -- 48 kHz stereo, 512-frame blocks, a
-1,048,576-voice pool with steal-percent 10, linear interpolation, a 16 x 4-zone
-bank of 32768-frame int16 samples, and dense bursts of note-ons (calibrated so
-the pool stays saturated and stealing fires every block).
+Default workload (c2, the plain method at saturation): 48 kHz stereo, 512-frame
+blocks, a 1,048,576-voice pool with steal-percent 10, linear interpolation, a
+16 x 4-zone bank of 32768-frame int16 samples, and dense bursts of note-ons
+(calibrated so the pool stays saturated and stealing fires every block).
+`--workload c1` runs the small CPU-runnable configuration over its whole
+10-second file; `--workload c3` the precision form (2,097,152 voices, stereo
+samples as two voices, cubic, +-2 semitone bends, resonant filters designed in
+FP64 on the device).
 
 A "step" is one block of the whole hot path on the device: steal select and
 marking, spawn, render, mixdown, locality sort and compaction. `--warmup` blocks
@@ -13,11 +17,13 @@ back to back (the timed region). Reported beside it:
 
 * plain: the same blocks through the reference-structured render and reduce,
   one synchronous Backend::render call per block, as the reference runs;
-* cpu_baseline: the CPU oracle (a restatement of src/cpu.rs, one thread)
-  rendering the first timed block from the same pool snapshot, whose output
-  is also the parity check (null in dB re the block peak);
+* cpu_baseline: the CPU oracle (a restatement of src/cpu.rs) rendering the
+  first timed block from the same pool snapshot, whose output is also the
+  parity check (null in dB re the block peak);
 * roofline: the render kernel's algorithmic bytes per launch over its mean
-  duration (HIP events on the backend's own stream), against 8 TB/s.
+  duration (HIP events on the backend's own stream), against 8 TB/s;
+* precision (c3): the first timed block against the oracle's FP64 render of
+  the same algorithm (f64 design, f64 voice arithmetic, f64 mix).
 """
 from __future__ import annotations
 
@@ -65,6 +71,41 @@ def render_voices(cfg, live0, blocks, stats):
     return np.array(out, np.float64)
 
 
+WORKLOADS = {
+    "c1": "plain method, small: 48 kHz stereo, 256-frame blocks, 4096-voice pool, steal-percent "
+          "25, linear interpolation, 16x4-zone bank of 32768-frame int16 samples, 10,000 "
+          "Poisson notes over 10 s (the whole file)",
+    "c2": "plain method at saturation: 48 kHz stereo, 512-frame blocks, 1,048,576-voice pool, "
+          "steal-percent 10, linear interpolation, 16x4-zone bank of 32768-frame int16 samples, "
+          "burst note-ons",
+    "c3": "precision form: 48 kHz stereo, 512-frame blocks, 2,097,152-voice pool (steal Oldest "
+          "10 %), stereo 65536-frame saw+noise samples as two hard-panned voices, cubic "
+          "interpolation, +-2 semitone bend random walks, lowpass Q log-uniform 0.5-20 designed "
+          "in FP64 on the device, compensated mixdown, burst note-ons",
+}
+
+
+def setup(name: str, args, seconds: float):
+    """(cfg, bank, builder, design_on_device) for a named workload."""
+    from kestrel_amd import workload as W
+    if name == "c1":
+        cfg = W.config_c1(compensated=bool(args.compensated), profile=not args.graph)
+        cfg, bank, notes = W.workload_c1(cfg=cfg)
+        return cfg, bank, W.BlockBuilder(cfg, bank, notes), False
+    if name == "c2":
+        cfg = W.config_c2(compensated=bool(args.compensated), profile=not args.graph)
+        _, bank, notes = W.workload_c2(args.seed or 0x4B530002, seconds=seconds, cfg=cfg,
+                                       base_rate_hz=500_000.0)
+        return cfg, bank, W.BlockBuilder(cfg, bank, notes), False
+    if name == "c3":
+        cfg = W.config_c3(profile=not args.graph)
+        _, bank, notes = W.workload_c3(args.seed or 0x4B530003, seconds=seconds, cfg=cfg,
+                                       base_rate_hz=500_000.0)
+        return cfg, bank, W.BlockBuilder(cfg, bank, notes,
+                                         bends=W.random_walk_bends(0x4B53, 2.0, 0.05)), True
+    raise SystemExit(f"unknown workload {name!r}; one of {sorted(WORKLOADS)}")
+
+
 def calibrate_copy():
     import torch
     n = 1 << 28
@@ -81,14 +122,17 @@ def calibrate_copy():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=320)
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed blocks (default: 200 for c2, 100 for c3, the whole file for c1)")
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed blocks that fill the pool (default 320; 0 for c1)")
     ap.add_argument("--compensated", type=int, default=1)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", action="store_true",
                     help="replay the block chain from a hipGraph (no per-pass events)")
-    ap.add_argument("--seed", type=int, default=0x4B530002)
+    ap.add_argument("--seed", type=int, default=None)
     ap.add_argument("--waves-per-group", type=int, default=None)
     ap.add_argument("--sort-digit-bits", type=int, default=None)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -103,23 +147,26 @@ def main():
         calibrate_copy()
         return
 
-    from kestrel_amd import workload as W
     from kestrel_amd.backend import HipSynth
 
-    K, WU = args.steps, args.warmup
-    cfg = W.config_c2(compensated=bool(args.compensated), profile=not args.graph)
+    wl = args.workload
+    K = args.steps if args.steps is not None else {"c1": 1875, "c2": 200, "c3": 100}[wl]
+    WU = args.warmup if args.warmup is not None else (0 if wl == "c1" else 320)
+    block_frames = 256 if wl == "c1" else 512
+    seconds = (WU + K + 4) * block_frames / 48000
+    t = time.time()
+    cfg, bank, builder, design = setup(wl, args, seconds)
     for knob in ("waves_per_group", "sort_digit_bits"):
         if getattr(args, knob) is not None:
             cfg = cfg.replace(**{knob: getattr(args, knob)})
-    seconds = (WU + K + 4) * cfg.block_frames / cfg.sample_rate
-    t = time.time()
-    _, bank, notes = W.workload_c2(args.seed, seconds=seconds, cfg=cfg,
-                                   base_rate_hz=500_000.0)
-    builder = W.BlockBuilder(cfg, bank, notes)
-    log(f"workload: {len(notes)} notes over {seconds:.2f} s, bank {bank.n_regions} regions, "
-        f"built in {time.time() - t:.1f} s")
+    log(f"workload {wl}: {len(builder.notes)} notes, bank {bank.n_regions} regions"
+        f"{' (stereo)' if bank.stereo else ''}, built in {time.time() - t:.1f} s")
 
     gpu = HipSynth(cfg, bank.pool, bank.params, bank.n_regions, bank.max_sample_len)
+    table = bank.params
+    if design:
+        gpu.design_params(0, bank.design)  # FP64 on the device, rounded once to f32
+        table = gpu.params_export(0)
     # ---- warmup: fill the pool (untimed) ----
     t = time.time()
     done = 0
@@ -160,7 +207,7 @@ def main():
     # ---- roofline of the render kernel ----
     step = (snapshot[3].astype(np.float64) * 2 ** 32 + snapshot[2]) / 2 ** 32
     mean_step = float(step.mean()) if step.size else 1.0
-    taps = 2
+    taps = {0: 1, 1: 2, 2: 4}[cfg.interpolation]
     spawned = np.array([min(len(b.cmds), cfg.max_voices) for b in timed], np.float64)
     bytes_block = V * (192.0 + 2.0 * (taps - 1) + 2.0 * F * mean_step) + 76.0 * spawned + 8.0 * F
     per_unit = float(bytes_block.sum() / voice_samples)
@@ -183,36 +230,70 @@ def main():
     # ---- CPU baseline + full-size parity on the first timed block ----
     cpu_baseline = None
     parity = {}
+    precision = None
     if not args.no_cpu:
         from oracle import oracle
-        o = oracle.Oracle(cfg, bank.pool, bank.params)
+        nb = K if wl == "c1" else 1  # c1: the whole file; else the first timed block
+        o = oracle.Oracle(cfg, bank.pool, table)
         o.set_sorted(bank.n_regions, bank.max_sample_len, True)
         o.pool_import(snapshot)
         o.set_counters(st0["stolen"], st0["dropped"])
-        blk = timed[0]
-        o.set_gates(blk.gates)
-        o.set_channels(blk.chans, *blk.flags)
-        o.spawn(blk.cmds)
-        tc = time.perf_counter()
-        oc = o.render()
-        tcpu = time.perf_counter() - tc
-        cpu_vs = V[0] * F / tcpu
+        oc = np.empty((nb, cfg.block_frames * 2), np.float64)
+        tcpu = 0.0
+        pool_ok = True
+        for i, blk in enumerate(timed[:nb]):
+            o.set_gates(blk.gates)
+            o.set_channels(blk.chans, *blk.flags)
+            o.spawn(blk.cmds)
+            tc = time.perf_counter()
+            oc[i] = o.render_f64()
+            tcpu += time.perf_counter() - tc
+            if i == 0:
+                oracle_pool = o.pool_export()
+        cpu_vs = float(V[:nb].sum()) * F / tcpu
         cpu_baseline = {"value": round(cpu_vs, 1), "unit": "voice-samples/s", "cores": 1,
                         "kind": "port",
-                        "sample": f"1 block of {F} frames at {int(V[0])} voices "
+                        "sample": f"{nb} block(s) of {F} frames at {int(V[:nb].mean())} voices "
                                   f"({tcpu:.1f} s, single-threaded restatement of src/cpu.rs)",
-                        "x_realtime": round(F / cfg.sample_rate / tcpu, 4)}
-        parity["null_db_vs_cpu"] = round(null_db(outs[0], oc), 1)
-        parity["stats_match_cpu"] = o.stats()["active_voices"] == stats[0]["active_voices"]
+                        "x_realtime": round(nb * F / cfg.sample_rate / tcpu, 4)}
+        parity["null_db_vs_cpu"] = round(null_db(outs[:nb], oc), 1)
+        parity["blocks_checked"] = nb
+        parity["stats_match_cpu"] = o.stats()["active_voices"] == stats[nb - 1]["active_voices"]
+        if nb > 1:
+            parity["pool_bit_exact_vs_cpu_at_end"] = bool(np.array_equal(o.pool_export(),
+                                                                         pool_final))
         log(f"cpu: {cpu_baseline}, null vs cpu {parity['null_db_vs_cpu']} dB")
-        oracle_pool = o.pool_export()
         del o
+        if design:
+            # The same block in FP64: f64 design, f64 voice state, f64 mix.
+            o64 = oracle.Oracle(cfg, bank.pool, table, precision=1)
+            o64.set_params64(bank.params_f64(cfg))
+            o64.set_sorted(bank.n_regions, bank.max_sample_len, True)
+            o64.pool_import(snapshot)
+            blk = timed[0]
+            o64.set_gates(blk.gates)
+            o64.set_channels(blk.chans, *blk.flags)
+            o64.spawn(blk.cmds)
+            ref = o64.render_f64()
+            peak = float(np.abs(ref).max())
+            dg = outs[0].astype(np.float64) - ref
+            dc = oc[0] - ref
+            to_db = lambda x: round(20 * math.log10(max(x, 1e-300) / max(peak, 1e-30)), 1)
+            precision = {"vs": "oracle FP64 render of the first timed block", "peak_ref": peak,
+                         "device_peak_err_db": to_db(float(np.abs(dg).max())),
+                         "device_rms_err_db": to_db(float(np.sqrt(np.mean(dg * dg)))),
+                         "cpu_f32_peak_err_db": to_db(float(np.abs(dc).max())),
+                         "cpu_f32_rms_err_db": to_db(float(np.sqrt(np.mean(dc * dc))))}
+            log(f"precision: {precision}")
+            del o64
 
     # ---- plain form: reference-structured kernels, one synchronous call per block ----
     plain = None
     if not args.no_plain:
         pcfg = cfg.replace(form=0, compensated=False, profile=False)
         p = HipSynth(pcfg, bank.pool, bank.params, bank.n_regions, bank.max_sample_len)
+        if design:
+            p.set_params_variant(0, table)
         p.pool_import(snapshot)
         p.set_counters(st0["stolen"], st0["dropped"])
         pouts = np.empty_like(outs)
@@ -240,9 +321,7 @@ def main():
         "n_gpus": 1, "steps": K, "warmup": WU, "ms_per_step": round(wall / K * 1e3, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "plain method at saturation: 48 kHz stereo, 512-frame blocks, "
-                               "1,048,576-voice pool, steal-percent 10, linear interpolation, "
-                               "16x4-zone bank of 32768-frame int16 samples, burst note-ons",
+        "config": {"workload": WORKLOADS[wl], "name": wl,
                    "block_frames": F, "max_voices": cfg.max_voices,
                    "mean_voices": round(float(V.mean()), 1),
                    "form": gpu.name(), "graph": bool(args.graph)},
@@ -252,6 +331,7 @@ def main():
         "plain": plain,
         "vs_plain": round(value / plain["value"], 3) if plain else None,
         "parity": parity,
+        "precision": precision,
     }
     gpu.close()
     print(json.dumps(line), flush=True)
