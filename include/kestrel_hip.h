@@ -161,6 +161,23 @@ int kst_run_staged(kst_backend *b, int use_graph);
  * per-block stats (nblocks entries, as stats() would read after each render). */
 int kst_staged_results(kst_backend *b, float *out, kst_block_stats *stats);
 
+/* ---- batching form: many renders, one launch per pass (no counterpart in the reference) ----
+ *
+ * A batch runs the staged blocks of several independent backends (each with
+ * its own config, bank and pool, all wave64 form on one device, all with the
+ * same `compensated`) together: block step by block step, every pass of the
+ * per-block chain is one launch for all members, so small renders share the
+ * GPU instead of each leaving it mostly idle. Members with fewer staged
+ * blocks idle once theirs are done. Results are identical to kst_run_staged
+ * on each member; read them with kst_staged_results. Destroy the batch before
+ * any of its members; a member must not be used concurrently with the batch. */
+typedef struct kst_batch kst_batch;
+int kst_batch_create(kst_backend *const *renders, size_t n, kst_batch **out);
+/* Runs every member's staged blocks. use_graph != 0 replays one captured
+ * hipGraph of the whole batch step. Returns when the device is done. */
+int kst_batch_run(kst_batch *batch, int use_graph);
+void kst_batch_destroy(kst_batch *batch);
+
 /* ---- precision form: FP64 coefficient design (no counterpart in the reference) ----
  *
  * The reference designs each RegionParams entry on the host in f32 (make_params

@@ -80,6 +80,9 @@ SIGNATURES = {
     "kst_stage_blocks": (i32, [P, u32, P, P, P, P, P]),
     "kst_run_staged": (i32, [P, i32]),
     "kst_staged_results": (i32, [P, P, P]),
+    "kst_batch_create": (i32, [P, sz, P]),
+    "kst_batch_run": (i32, [P, i32]),
+    "kst_batch_destroy": (None, [P]),
     "kst_design_params": (i32, [P, u32, P, sz]),
     "kst_params_export": (i32, [P, u32, P, sz]),
     "kst_pool_export": (i64, [P, P, sz]),

@@ -148,3 +148,32 @@ class HipSynth:
 
     def set_counters(self, stolen: int, dropped: int) -> None:
         check(self._lib.kst_set_counters(self._h, int(stolen), int(dropped)))
+
+
+class HipBatch:
+    """Several HipSynth renders run together, block step by block step, with
+    every pass of the chain one launch for all of them (kst_batch_*). Stage
+    each member's blocks first (HipSynth.stage_blocks); read results per member
+    with HipSynth.staged_results."""
+
+    def __init__(self, renders: list):
+        self._lib = abi.load()
+        self.renders = list(renders)
+        arr = (C.c_void_p * len(self.renders))(*[r._h.value for r in self.renders])
+        h = C.c_void_p()
+        check(self._lib.kst_batch_create(arr, len(self.renders), C.byref(h)))
+        self._h = h
+
+    def run(self, use_graph: bool = True) -> None:
+        check(self._lib.kst_batch_run(self._h, int(use_graph)))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.kst_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -338,6 +338,7 @@ struct kst_backend {
         a.partials = partials.p;
         a.fade_tab = fade_tab.p;
         a.fade_lds = fade_lds;
+        a.nblocks = std::max(staged, 1u);
         a.rsum = rsum.p;
         a.render_groups = render_groups;
         a.waves_per_group = cfg.waves_per_group;
@@ -886,6 +887,195 @@ int kst_staged_results(kst_backend *b, float *out, kst_block_stats *stats) {
             std::memcpy(&stats[i].peak, &r[i].peak_bits, 4);
             stats[i].pad = 0;
         }
+    }
+    return KST_OK;
+}
+
+// ---- batches: many renders, one launch per pass ----
+
+struct kst_batch {
+    struct Group {
+        uint32_t first = 0, count = 0, interp = 0, wpg = 1, groups = 1, fade_lds = 0;
+        bool chan = false, clamp = false;
+    };
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<kst_backend *> r;      // launch order: grouped by render variant
+    std::vector<Group> groups;
+    DevBuf<Args> args;
+    uint32_t nr = 0, sel_grid = 1, spawn_grid = 1, sort_grid = 1, gather_grid = 1;
+    uint32_t passes = 1, max_radix = 2, max_two_f = 2, max_rows = 1;
+    bool comp = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+
+    ~kst_batch() {
+        (void)hipSetDevice(device);
+        drop_graph();
+        args.release();
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+    // One block of every member: the chain of kst_backend::launch_block with
+    // each pass launched once for all members.
+    void launch_step() {
+        const Args *aa = args.p;
+        hipStream_t s = stream;
+        launch_batch_begin(aa, nr, s);
+        launch_batch_select(aa, nr, sel_grid, 3, s);
+        launch_batch_spawn(aa, nr, spawn_grid, s);
+        for (const Group &g : groups)
+            launch_batch_render(aa + g.first, g.count, g.interp, g.chan, g.clamp, g.wpg, g.groups,
+                                g.fade_lds, s);
+        launch_batch_reduce(aa, nr, comp, max_two_f, max_rows, s);
+        launch_batch_sort(aa, nr, sort_grid, gather_grid, passes, max_radix, s);
+        launch_batch_end(aa, nr, s);
+    }
+};
+
+int kst_batch_create(kst_backend *const *renders, size_t n, kst_batch **out) {
+    if (!renders || !out || n == 0) return fail(KST_EINVAL, "a batch needs at least one render");
+    if (n > 65535) return fail(KST_EINVAL, "%zu renders in one batch (at most 65535)", n);
+    *out = nullptr;
+    const kst_backend *r0 = renders[0];
+    for (size_t i = 0; i < n; ++i) {
+        const kst_backend *b = renders[i];
+        if (!b) return fail(KST_EINVAL, "render %zu is null", i);
+        if (!b->fast) return fail(KST_EINVAL, "render %zu is not the wave64 form", i);
+        if (b->device != r0->device)
+            return fail(KST_EINVAL, "render %zu is on device %d, the batch on %d", i, b->device,
+                        r0->device);
+        if ((b->cfg.compensated != 0) != (r0->cfg.compensated != 0))
+            return fail(KST_EINVAL, "render %zu differs in compensated", i);
+        for (size_t j = 0; j < i; ++j)
+            if (renders[j] == b) return fail(KST_EINVAL, "render %zu appears twice", i);
+    }
+    auto *bt = new kst_batch();
+    bt->device = r0->device;
+    bt->comp = r0->cfg.compensated != 0;
+    bt->nr = (uint32_t)n;
+    bt->r.assign(renders, renders + n);
+    if (hipSetDevice(bt->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&bt->stream, hipStreamNonBlocking) != hipSuccess ||
+        bt->args.alloc(n) != hipSuccess) {
+        delete bt;
+        return fail(KST_EDEVICE, "creating the batch stream and table");
+    }
+    *out = bt;
+    return KST_OK;
+}
+
+void kst_batch_destroy(kst_batch *bt) { delete bt; }
+
+int kst_batch_run(kst_batch *bt, int use_graph) {
+    if (!bt) return fail(KST_EINVAL, "null argument");
+    HIPCHK(hipSetDevice(bt->device));
+    // Group members by render variant (the render kernel's template), largest
+    // first inside a group is not needed: every grid covers the largest.
+    auto key = [](const kst_backend *b) {
+        return (b->cfg.interpolation << 3) | ((b->staged_chan ? 1u : 0u) << 2) |
+               ((b->bank_clamp ? 1u : 0u) << 1);
+    };
+    std::vector<kst_backend *> order(bt->r);
+    std::stable_sort(order.begin(), order.end(), [&](const kst_backend *x, const kst_backend *y) {
+        return std::make_pair(key(x), x->cfg.waves_per_group) <
+               std::make_pair(key(y), y->cfg.waves_per_group);
+    });
+    std::vector<kst_batch::Group> groups;
+    uint32_t steps = 0;
+    uint32_t sel = 1, spawn = 1, sort = 1, gather = 1, passes = 1, radix = 2, two_f = 2, rows = 1;
+    std::vector<Args> host(order.size());
+    for (size_t i = 0; i < order.size(); ++i) {
+        kst_backend *b = order[i];
+        if (!b->staged) return fail(KST_ESTATE, "a batch member has no staged blocks");
+        HIPCHK(hipStreamSynchronize(b->stream));
+        const uint32_t n = b->slots;
+        host[i] = b->args(b->render_groups_for(n));
+        steps = std::max(steps, b->staged);
+        sel = std::max(sel, std::max(1u, std::min(std::min(b->cus, SEL_MAX_BLOCKS),
+                                                  (b->cfg.max_voices + SEL_THREADS * 4 - 1) /
+                                                      (SEL_THREADS * 4))));
+        spawn = std::max(spawn, b->per_voice_grid(n));
+        sort = std::max(sort, b->sort_grid(n));
+        gather = std::max(gather, b->per_voice_grid(n));
+        passes = std::max(passes, b->sort_passes);
+        radix = std::max(radix, 1u << b->digit_bits);
+        two_f = std::max(two_f, 2 * b->cfg.block_frames);
+        rows = std::max(rows, (n + 63) / 64);
+        const bool chan = b->staged_chan;
+        if (groups.empty() || groups.back().interp != b->cfg.interpolation ||
+            groups.back().chan != chan || groups.back().clamp != (b->bank_clamp != 0) ||
+            groups.back().wpg != b->cfg.waves_per_group) {
+            kst_batch::Group g;
+            g.first = (uint32_t)i;
+            g.interp = b->cfg.interpolation;
+            g.chan = chan;
+            g.clamp = b->bank_clamp != 0;
+            g.wpg = b->cfg.waves_per_group;
+            groups.push_back(g);
+        }
+        kst_batch::Group &g = groups.back();
+        g.count += 1;
+        g.groups = std::max(g.groups, b->render_groups_for(n));
+        g.fade_lds = std::max(g.fade_lds, b->fade_lds);
+    }
+    const bool same_shape =
+        bt->exec && order == bt->r && groups.size() == bt->groups.size() &&
+        std::equal(groups.begin(), groups.end(), bt->groups.begin(),
+                   [](const kst_batch::Group &x, const kst_batch::Group &y) {
+                       return x.first == y.first && x.count == y.count && x.interp == y.interp &&
+                              x.chan == y.chan && x.clamp == y.clamp && x.wpg == y.wpg &&
+                              x.groups == y.groups && x.fade_lds == y.fade_lds;
+                   }) &&
+        sel == bt->sel_grid && spawn == bt->spawn_grid && sort == bt->sort_grid &&
+        gather == bt->gather_grid && passes == bt->passes && radix == bt->max_radix &&
+        two_f == bt->max_two_f && rows == bt->max_rows;
+    if (!same_shape) bt->drop_graph();
+    bt->r = order;
+    bt->groups = groups;
+    bt->sel_grid = sel;
+    bt->spawn_grid = spawn;
+    bt->sort_grid = sort;
+    bt->gather_grid = gather;
+    bt->passes = passes;
+    bt->max_radix = radix;
+    bt->max_two_f = two_f;
+    bt->max_rows = rows;
+    hipStream_t s = bt->stream;
+    HIPCHK(hipMemcpyAsync(bt->args.p, host.data(), host.size() * sizeof(Args),
+                          hipMemcpyHostToDevice, s));
+    uint32_t zero = 0;
+    for (kst_backend *b : order)
+        HIPCHK(hipMemcpyAsync(&b->st.p->block, &zero, 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (use_graph) {
+        if (!bt->exec) {
+            HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            bt->launch_step();
+            hipError_t e = hipStreamEndCapture(s, &bt->graph);
+            if (e != hipSuccess) return fail(KST_EDEVICE, "batch graph capture: %s", hipGetErrorString(e));
+            HIPCHK(hipGraphInstantiate(&bt->exec, bt->graph, nullptr, nullptr, 0));
+        }
+        for (uint32_t i = 0; i < steps; ++i) HIPCHK(hipGraphLaunch(bt->exec, s));
+    } else {
+        for (uint32_t i = 0; i < steps; ++i) bt->launch_step();
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    // Every member's host mirror from its last block's result row.
+    for (kst_backend *b : order) {
+        BlockResult res;
+        HIPCHK(hipMemcpy(&res, b->g_results.p + (b->staged - 1), sizeof res, hipMemcpyDeviceToHost));
+        b->live = (uint32_t)res.active_voices;
+        if (b->staged & 1) b->parity ^= 1;
+        b->stolen = res.stolen;
+        b->dropped = res.dropped;
+        std::memcpy(&b->peak, &res.peak_bits, 4);
     }
     return KST_OK;
 }

@@ -77,7 +77,7 @@ struct DevState {
     uint32_t live;    // compacted live count at the start of the block
     uint32_t parity;  // which pool buffer holds the live pool
     uint32_t block;   // index of the block within the staged inputs
-    uint32_t pad0;
+    uint32_t idle;    // no staged input left for this block: every pass is a no-op
     unsigned long long stolen, dropped;  // cumulative
     // Derived per block by k_begin (src/gpu/mod.rs:1015-1040).
     uint32_t want, steal_k, spawn_count, live_total;
@@ -119,6 +119,7 @@ struct Args {
     float *out;               // [nblocks][2F]
     BlockResult *results;     // [nblocks]
     // Config.
+    uint32_t nblocks;  // block slots holding inputs; a block at or past this idles
     uint32_t F, gate_frames, max_voices, max_steal, steal_span, fade;
     const float *fade_tab;  // [fade]: 1 - d / fade, as the per-frame fade computes it
     uint32_t fade_lds;      // entries of fade_tab staged in LDS (0: too long to stage)
@@ -163,6 +164,22 @@ void launch_end(const Args &a, hipStream_t s);
 bool fast_render_supported(uint32_t F, uint32_t gate_frames, uint32_t interp);
 size_t fast_render_lds_bytes(uint32_t waves_per_group, uint32_t fade_lds);
 void launch_render_fast(const Args &a, bool chan, hipStream_t s);
+
+// A batch of renders in one launch per pass: `aa` is a device array of `nr`
+// Args (the render's own buffers and config), and blockIdx.z picks the
+// render. Grids are sized for the largest render; smaller ones return early.
+void launch_batch_begin(const Args *aa, uint32_t nr, hipStream_t s);
+void launch_batch_select(const Args *aa, uint32_t nr, uint32_t grid, uint32_t rounds,
+                         hipStream_t s);
+void launch_batch_spawn(const Args *aa, uint32_t nr, uint32_t grid, hipStream_t s);
+void launch_batch_render(const Args *aa, uint32_t nr, uint32_t interp, bool chan, bool clamp,
+                         uint32_t waves_per_group, uint32_t groups, uint32_t fade_lds,
+                         hipStream_t s);
+void launch_batch_reduce(const Args *aa, uint32_t nr, bool comp, uint32_t max_two_f,
+                         uint32_t max_rows, hipStream_t s);
+void launch_batch_sort(const Args *aa, uint32_t nr, uint32_t grid, uint32_t gather_grid,
+                       uint32_t passes, uint32_t max_radix, hipStream_t s);
+void launch_batch_end(const Args *aa, uint32_t nr, hipStream_t s);
 void launch_design(const kst_region_design *in, uint32_t n, kst_region_params *out, double sr,
                    bool exp_decay, bool exp_release, float env_floor, hipStream_t s);
 void launch_reduce_fast(const Args &a, bool comp, uint32_t max_rows, hipStream_t s);

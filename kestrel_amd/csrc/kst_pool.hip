@@ -30,10 +30,20 @@ __device__ __forceinline__ uint64_t steal_key(const Args &a, const uint32_t *poo
     return ((uint64_t)q << 48) | (id & 0x0000FFFFFFFFFFFFull);
 }
 
-__global__ void k_begin(Args a) {
+__device__ __forceinline__ void begin_body(const Args &a) {
     DevState *st = a.st;
     const uint32_t b = st->block;
     if (threadIdx.x != 0) return;
+    if (b >= a.nblocks) {
+        // A batch member whose blocks are done: nothing to select, spawn,
+        // render or sort, and the pool and counters stay as they are.
+        st->idle = 1;
+        st->steal_k = 0;
+        st->spawn_count = 0;
+        st->live_total = 0;
+        return;
+    }
+    st->idle = 0;
     // src/gpu/mod.rs:1015-1040 (and src/cpu.rs:330-384, which agrees).
     const uint64_t cap = a.max_voices;
     const uint64_t live = st->live;
@@ -63,6 +73,11 @@ __global__ void k_begin(Args a) {
     st->sel_count = (uint32_t)live;
     st->sel_done = live <= SEL_CAP;  // few enough for the final workgroup alone
     st->cand_n = 0;
+}
+__global__ void k_begin(Args a) { begin_body(a); }
+// The same pass over a batch of renders: blockIdx.z picks the render.
+__global__ void kb_begin(const Args *__restrict__ aa) {
+    begin_body(aa[blockIdx.z]);
 }
 
 // ---------------------------------------------------------------------------
@@ -157,7 +172,7 @@ __device__ void sel_choose(DevState *st, const uint32_t *cnt,
 
 // One MSB-first round over the pool: per-workgroup bin statistics of the
 // candidates' next digit, stored without atomics.
-__global__ void __launch_bounds__(SEL_THREADS) k_sel_hist(Args a) {
+__device__ __forceinline__ void sel_hist_body(const Args &a) {
     DevState *st = a.st;
     if (st->steal_k == 0 || st->sel_done) return;
     __shared__ uint32_t cnt[SEL_BINS];
@@ -195,11 +210,16 @@ __global__ void __launch_bounds__(SEL_THREADS) k_sel_hist(Args a) {
         a.sel_max[o] = mx[i];
     }
 }
+__global__ void __launch_bounds__(SEL_THREADS) k_sel_hist(Args a) { sel_hist_body(a); }
+// The same pass over a batch of renders: blockIdx.z picks the render.
+__global__ void __launch_bounds__(SEL_THREADS) kb_sel_hist(const Args *__restrict__ aa) {
+    sel_hist_body(aa[blockIdx.z]);
+}
 
 // One workgroup sums the per-workgroup counts (four threads per bin, each over
 // a quarter of the rows) and picks the bin; only that bin's smallest and
 // largest keys are then read back from every row.
-__global__ void __launch_bounds__(SEL_THREADS) k_sel_refine(Args a, uint32_t blocks) {
+__device__ __forceinline__ void sel_refine_body(const Args &a, uint32_t blocks) {
     DevState *st = a.st;
     if (st->steal_k == 0 || st->sel_done) return;
     constexpr uint32_t Q = SEL_THREADS / SEL_BINS;
@@ -268,6 +288,11 @@ __global__ void __launch_bounds__(SEL_THREADS) k_sel_refine(Args a, uint32_t blo
         st->sel_done = n <= SEL_CAP || top == 0;
     }
 }
+__global__ void __launch_bounds__(SEL_THREADS) k_sel_refine(Args a, uint32_t blocks) { sel_refine_body(a, blocks); }
+// The same pass over a batch of renders: blockIdx.z picks the render.
+__global__ void __launch_bounds__(SEL_THREADS) kb_sel_refine(const Args *__restrict__ aa, uint32_t blocks) {
+    sel_refine_body(aa[blockIdx.z], blocks);
+}
 
 __device__ __forceinline__ void mark_victim(const Args &a, uint32_t *pool, uint32_t i) {
     // src/cpu.rs:365-369 / shaders/compact.wgsl:174-175
@@ -277,7 +302,7 @@ __device__ __forceinline__ void mark_victim(const Args &a, uint32_t *pool, uint3
 
 // Every key below the candidates is a victim; the candidates are collected
 // (one global reservation per workgroup).
-__global__ void __launch_bounds__(SEL_THREADS) k_sel_collect(Args a) {
+__device__ __forceinline__ void sel_collect_body(const Args &a) {
     DevState *st = a.st;
     if (st->steal_k == 0) return;
     __shared__ uint32_t s_n, s_base;
@@ -309,11 +334,16 @@ __global__ void __launch_bounds__(SEL_THREADS) k_sel_collect(Args a) {
         }
     }
 }
+__global__ void __launch_bounds__(SEL_THREADS) k_sel_collect(Args a) { sel_collect_body(a); }
+// The same pass over a batch of renders: blockIdx.z picks the render.
+__global__ void __launch_bounds__(SEL_THREADS) kb_sel_collect(const Args *__restrict__ aa) {
+    sel_collect_body(aa[blockIdx.z]);
+}
 
 // Finish on the candidates with more rounds in one workgroup until the bin
 // holds the k-th key alone; keys below each chosen bin become victims as the
 // candidate list is narrowed, and the last one standing is the k-th.
-__global__ void __launch_bounds__(SEL_THREADS) k_sel_final(Args a) {
+__device__ __forceinline__ void sel_final_body(const Args &a) {
     DevState *st = a.st;
     if (st->steal_k == 0) return;
     __shared__ uint32_t cnt[SEL_BINS], scan[SEL_BINS];
@@ -372,10 +402,15 @@ __global__ void __launch_bounds__(SEL_THREADS) k_sel_final(Args a) {
     }
     if (threadIdx.x == 0 && n == 1 && st->sel_k == 1) mark_victim(a, pool, a.cand_slot[0]);
 }
+__global__ void __launch_bounds__(SEL_THREADS) k_sel_final(Args a) { sel_final_body(a); }
+// The same pass over a batch of renders: blockIdx.z picks the render.
+__global__ void __launch_bounds__(SEL_THREADS) kb_sel_final(const Args *__restrict__ aa) {
+    sel_final_body(aa[blockIdx.z]);
+}
 
 // shaders/spawn.wgsl:35-78; thinning per src/voice.rs:72-75 when the block is
 // oversubscribed under AdmitRule::Even (src/gpu/mod.rs:1056-1071).
-__global__ void __launch_bounds__(256) k_spawn(Args a) {
+__device__ __forceinline__ void spawn_body(const Args &a) {
     DevState *st = a.st;
     const uint32_t n = st->spawn_count;
     if (n == 0) return;
@@ -416,6 +451,11 @@ __global__ void __launch_bounds__(256) k_spawn(Args a) {
         pool[F_STOP_REL * c + d] = 0u;
     }
 }
+__global__ void __launch_bounds__(256) k_spawn(Args a) { spawn_body(a); }
+// The same pass over a batch of renders: blockIdx.z picks the render.
+__global__ void __launch_bounds__(256) kb_spawn(const Args *__restrict__ aa) {
+    spawn_body(aa[blockIdx.z]);
+}
 
 // ---------------------------------------------------------------------------
 // Locality sort / compaction: stable LSD radix sort of (key, slot) pairs over
@@ -431,7 +471,8 @@ __device__ __forceinline__ bool key_alive(const Args &a, uint32_t key) {
 // Per-tile digit counts of pass p over that pass's input order (the previous
 // pass's output). Each tile's keys are loaded up front; counts go to LDS and
 // then to the tile's column of sort_hist, with no global atomics.
-__global__ void __launch_bounds__(SORT_TPB) k_sort_hist(Args a, uint32_t p) {
+__device__ __forceinline__ void sort_hist_body(const Args &a, uint32_t p) {
+    if (p >= a.sort_passes) return;  // a batch runs the most passes any member needs
     DevState *st = a.st;
     const uint32_t n = st->live_total;
     const uint32_t tiles = (n + SORT_TILE - 1) / SORT_TILE;
@@ -459,6 +500,11 @@ __global__ void __launch_bounds__(SORT_TPB) k_sort_hist(Args a, uint32_t p) {
         __syncthreads();
     }
 }
+__global__ void __launch_bounds__(SORT_TPB) k_sort_hist(Args a, uint32_t p) { sort_hist_body(a, p); }
+// The same pass over a batch of renders: blockIdx.z picks the render.
+__global__ void __launch_bounds__(SORT_TPB) kb_sort_hist(const Args *__restrict__ aa, uint32_t p) {
+    sort_hist_body(aa[blockIdx.z], p);
+}
 
 // Exclusive scan of `v` (one value per thread of a 256-thread workgroup) in
 // `buf`; returns the exclusive prefix and leaves the total in *total.
@@ -481,7 +527,8 @@ __device__ __forceinline__ uint32_t block_exscan256(uint32_t v, uint32_t *buf, u
 // One workgroup per (pass, digit): each tile's offset within the digit is the
 // same digit's count in the tiles before it; the digit's total goes to
 // sort_tot for the scatter to place the digits.
-__global__ void __launch_bounds__(256) k_sort_scan(Args a, uint32_t p) {
+__device__ __forceinline__ void sort_scan_body(const Args &a, uint32_t p) {
+    if (p >= a.sort_passes || blockIdx.x >= (1u << a.digit_bits)) return;
     DevState *st = a.st;
     const uint32_t n = st->live_total;
     const uint32_t tiles = (n + SORT_TILE - 1) / SORT_TILE;
@@ -500,6 +547,11 @@ __global__ void __launch_bounds__(256) k_sort_scan(Args a, uint32_t p) {
     }
     if (threadIdx.x == 0) a.sort_tot[p * radix + d] = running;
 }
+__global__ void __launch_bounds__(256) k_sort_scan(Args a, uint32_t p) { sort_scan_body(a, p); }
+// The same pass over a batch of renders: blockIdx.z picks the render.
+__global__ void __launch_bounds__(256) kb_sort_scan(const Args *__restrict__ aa, uint32_t p) {
+    sort_scan_body(aa[blockIdx.z], p);
+}
 
 // Stable scatter of one digit pass. Items of a tile are ranked in index
 // order: rounds of 256 consecutive items, waves in order within a round,
@@ -507,7 +559,8 @@ __global__ void __launch_bounds__(256) k_sort_scan(Args a, uint32_t p) {
 // first lane in each wave publishes the wave's count for that digit and
 // clears it again after the round, so the per-wave table stays zero between
 // rounds without a full sweep.
-__global__ void __launch_bounds__(SORT_TPB) k_sort_scatter(Args a, uint32_t p) {
+__device__ __forceinline__ void sort_scatter_body(const Args &a, uint32_t p) {
+    if (p >= a.sort_passes) return;  // a batch runs the most passes any member needs
     DevState *st = a.st;
     const uint32_t n = st->live_total;
     const uint32_t tiles = (n + SORT_TILE - 1) / SORT_TILE;
@@ -587,12 +640,17 @@ __global__ void __launch_bounds__(SORT_TPB) k_sort_scatter(Args a, uint32_t p) {
         __syncthreads();
     }
 }
+__global__ void __launch_bounds__(SORT_TPB) k_sort_scatter(Args a, uint32_t p) { sort_scatter_body(a, p); }
+// The same pass over a batch of renders: blockIdx.z picks the render.
+__global__ void __launch_bounds__(SORT_TPB) kb_sort_scatter(const Args *__restrict__ aa, uint32_t p) {
+    sort_scatter_body(aa[blockIdx.z], p);
+}
 
 // shaders/sort.wgsl:376-400 / compact.wgsl:116-146: survivors move to the other
 // buffer in sorted (or original) order; per-block fields are cleared. Dead
 // voices sort after every survivor, so the last survivor's thread records the
 // new live count.
-__global__ void __launch_bounds__(256) k_gather(Args a) {
+__device__ __forceinline__ void gather_body(const Args &a) {
     DevState *st = a.st;
     const uint32_t n = st->live_total;
     const uint32_t *perm = sel2(a.idx, a.sort_passes & 1);
@@ -617,9 +675,15 @@ __global__ void __launch_bounds__(256) k_gather(Args a) {
         for (uint32_t f = 0; f < VOICE_FIELDS; ++f) dst[f * c + j] = v[f];
     }
 }
+__global__ void __launch_bounds__(256) k_gather(Args a) { gather_body(a); }
+// The same pass over a batch of renders: blockIdx.z picks the render.
+__global__ void __launch_bounds__(256) kb_gather(const Args *__restrict__ aa) {
+    gather_body(aa[blockIdx.z]);
+}
 
-__global__ void k_end(Args a) {
+__device__ __forceinline__ void end_body(const Args &a) {
     DevState *st = a.st;
+    if (st->idle) return;
     BlockResult r;
     r.active_voices = st->live_new;
     r.stolen = st->stolen;
@@ -630,6 +694,11 @@ __global__ void k_end(Args a) {
     st->live = st->live_new;
     st->parity ^= 1;
     st->block += 1;
+}
+__global__ void k_end(Args a) { end_body(a); }
+// The same pass over a batch of renders: blockIdx.z picks the render.
+__global__ void kb_end(const Args *__restrict__ aa) {
+    end_body(aa[blockIdx.z]);
 }
 
 }  // namespace
@@ -660,5 +729,37 @@ void launch_sort(const Args &a, uint32_t grid, uint32_t gather_grid, hipStream_t
 }
 
 void launch_end(const Args &a, hipStream_t s) { k_end<<<1, 1, 0, s>>>(a); }
+
+void launch_batch_begin(const Args *aa, uint32_t nr, hipStream_t s) {
+    kb_begin<<<dim3(1, 1, nr), 256, 0, s>>>(aa);
+}
+
+void launch_batch_select(const Args *aa, uint32_t nr, uint32_t grid, uint32_t rounds,
+                         hipStream_t s) {
+    for (uint32_t r = 0; r < rounds; ++r) {
+        kb_sel_hist<<<dim3(grid, 1, nr), SEL_THREADS, 0, s>>>(aa);
+        kb_sel_refine<<<dim3(1, 1, nr), SEL_THREADS, 0, s>>>(aa, grid);
+    }
+    kb_sel_collect<<<dim3(grid, 1, nr), SEL_THREADS, 0, s>>>(aa);
+    kb_sel_final<<<dim3(1, 1, nr), SEL_THREADS, 0, s>>>(aa);
+}
+
+void launch_batch_spawn(const Args *aa, uint32_t nr, uint32_t grid, hipStream_t s) {
+    kb_spawn<<<dim3(grid, 1, nr), 256, 0, s>>>(aa);
+}
+
+void launch_batch_sort(const Args *aa, uint32_t nr, uint32_t grid, uint32_t gather_grid,
+                       uint32_t passes, uint32_t max_radix, hipStream_t s) {
+    for (uint32_t p = 0; p < passes; ++p) {
+        kb_sort_hist<<<dim3(grid, 1, nr), SORT_TPB, 0, s>>>(aa, p);
+        kb_sort_scan<<<dim3(max_radix, 1, nr), 256, 0, s>>>(aa, p);
+        kb_sort_scatter<<<dim3(grid, 1, nr), SORT_TPB, 0, s>>>(aa, p);
+    }
+    kb_gather<<<dim3(gather_grid, 1, nr), 256, 0, s>>>(aa);
+}
+
+void launch_batch_end(const Args *aa, uint32_t nr, hipStream_t s) {
+    kb_end<<<dim3(1, 1, nr), 1, 0, s>>>(aa);
+}
 
 }  // namespace kst

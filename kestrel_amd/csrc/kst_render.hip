@@ -686,7 +686,7 @@ __device__ __forceinline__ void two_sum(float a, float b, float &s, float &e) {
 // sums into its own row of the partial buffer (row = batch), so no two waves
 // share an accumulator and the summation order never depends on scheduling.
 template <int INTERP, bool CHAN, bool CLAMP>
-__global__ void __launch_bounds__(256) k_render_fast(Args a) {
+__device__ __forceinline__ void render_fast_body(const Args &a) {
     extern __shared__ float lds[];
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = threadIdx.x >> 6;
@@ -757,13 +757,20 @@ __global__ void __launch_bounds__(256) k_render_fast(Args a) {
     }
     if (v.live) store_voice(a, pool, vi, v);
 }
+template <int INTERP, bool CHAN, bool CLAMP>
+__global__ void __launch_bounds__(256) k_render_fast(Args a) { render_fast_body<INTERP, CHAN, CLAMP>(a); }
+// The same pass over a batch of renders: blockIdx.z picks the render.
+template <int INTERP, bool CHAN, bool CLAMP>
+__global__ void __launch_bounds__(256) kb_render_fast(const Args *__restrict__ aa) {
+    render_fast_body<INTERP, CHAN, CLAMP>(aa[blockIdx.z]);
+}
 
 // Stage 1 of the mixdown: column sums of the batch rows in fixed order, one
 // chunk of REDUCE_ROWS rows per workgroup (64 columns x 4 row phases), written
 // as (hi, lo) pairs. With COMP the sums are compensated (two-sum), so the
 // cross-batch sum of a million voices keeps its low-order bits.
 template <bool COMP>
-__global__ void __launch_bounds__(256) k_reduce_rows(Args a) {
+__device__ __forceinline__ void reduce_rows_body(const Args &a) {
     DevState *st = a.st;
     const uint32_t two_f = 2 * a.F;
     const uint32_t rows = (st->live_total + 63) / 64;
@@ -819,12 +826,20 @@ __global__ void __launch_bounds__(256) k_reduce_rows(Args a) {
         a.rsum[((size_t)chunk * two_f + col) * 2 + 1] = l;
     }
 }
+template <bool COMP>
+__global__ void __launch_bounds__(256) k_reduce_rows(Args a) { reduce_rows_body<COMP>(a); }
+// The same pass over a batch of renders: blockIdx.z picks the render.
+template <bool COMP>
+__global__ void __launch_bounds__(256) kb_reduce_rows(const Args *__restrict__ aa) {
+    reduce_rows_body<COMP>(aa[blockIdx.z]);
+}
 
 // Stage 2: the chunks in order, one thread per output sample, then the block
 // peak (src/cpu.rs:615-625).
 template <bool COMP>
-__global__ void __launch_bounds__(256) k_reduce_final(Args a) {
+__device__ __forceinline__ void reduce_final_body(const Args &a) {
     DevState *st = a.st;
+    if (st->idle) return;
     const uint32_t two_f = 2 * a.F;
     const uint32_t col = blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= two_f) return;
@@ -857,6 +872,13 @@ __global__ void __launch_bounds__(256) k_reduce_final(Args a) {
     a.out[(size_t)st->block * two_f + col] = v;
     const float m = fabsf(v);
     if (m == m) atomicMax(&st->peak_bits, __float_as_uint(m));
+}
+template <bool COMP>
+__global__ void __launch_bounds__(256) k_reduce_final(Args a) { reduce_final_body<COMP>(a); }
+// The same pass over a batch of renders: blockIdx.z picks the render.
+template <bool COMP>
+__global__ void __launch_bounds__(256) kb_reduce_final(const Args *__restrict__ aa) {
+    reduce_final_body<COMP>(aa[blockIdx.z]);
 }
 
 // ---------------------------------------------------------------------------
@@ -990,6 +1012,37 @@ void launch_render_fast(const Args &a, bool chan, hipStream_t s) {
     KST_FAST(1, true)
     KST_FAST(2, true)
 #undef KST_FAST
+}
+
+void launch_batch_render(const Args *aa, uint32_t nr, uint32_t interp, bool chan, bool clamp,
+                         uint32_t waves_per_group, uint32_t groups, uint32_t fade_lds,
+                         hipStream_t s) {
+    const dim3 grid((groups + 7) & ~7u, 1, nr);  // a multiple of 8: XCD = blockIdx.x % 8
+    const uint32_t threads = waves_per_group * 64;
+    const size_t lds = fast_render_lds_bytes(waves_per_group, fade_lds);
+#define KST_BFAST(I, CH, CL)                                                                \
+    if (interp == I && chan == CH && clamp == CL) {                                         \
+        kb_render_fast<I, CH, CL><<<grid, threads, lds, s>>>(aa);                          \
+        return;                                                                             \
+    }
+    KST_BFAST(0, false, false) KST_BFAST(1, false, false) KST_BFAST(2, false, false)
+    KST_BFAST(0, true, false) KST_BFAST(1, true, false) KST_BFAST(2, true, false)
+    KST_BFAST(0, false, true) KST_BFAST(1, false, true) KST_BFAST(2, false, true)
+    KST_BFAST(0, true, true) KST_BFAST(1, true, true) KST_BFAST(2, true, true)
+#undef KST_BFAST
+}
+
+void launch_batch_reduce(const Args *aa, uint32_t nr, bool comp, uint32_t max_two_f,
+                         uint32_t max_rows, hipStream_t s) {
+    const dim3 g1((max_two_f + 63) / 64, (max_rows + REDUCE_ROWS - 1) / REDUCE_ROWS, nr);
+    const dim3 g2((max_two_f + 255) / 256, 1, nr);
+    if (comp) {
+        kb_reduce_rows<true><<<g1, 256, 0, s>>>(aa);
+        kb_reduce_final<true><<<g2, 256, 0, s>>>(aa);
+    } else {
+        kb_reduce_rows<false><<<g1, 256, 0, s>>>(aa);
+        kb_reduce_final<false><<<g2, 256, 0, s>>>(aa);
+    }
 }
 
 void launch_reduce_fast(const Args &a, bool comp, uint32_t max_rows, hipStream_t s) {
