@@ -78,6 +78,10 @@ WORKLOADS = {
     "c2": "plain method at saturation: 48 kHz stereo, 512-frame blocks, 1,048,576-voice pool, "
           "steal-percent 10, linear interpolation, 16x4-zone bank of 32768-frame int16 samples, "
           "burst note-ons",
+    "c4": "batching form: 64 independent renders (pools 4,096-524,288 log-uniform, 1e3-5e6 "
+          "notes over 5-60 s, 4-64 instruments, 4k-128k-frame samples, linear or cubic, block "
+          "length and waves per workgroup from the autotuner), one launch per pass for all, "
+          "hipGraph-replayed block step",
     "c3": "precision form: 48 kHz stereo, 512-frame blocks, 2,097,152-voice pool (steal Oldest "
           "10 %), stereo 65536-frame saw+noise samples as two hard-panned voices, cubic "
           "interpolation, +-2 semitone bend random walks, lowpass Q log-uniform 0.5-20 designed "
@@ -119,6 +123,124 @@ def calibrate_copy():
           flush=True)
 
 
+def run_c4(args):
+    """The batching form: every member's window in one batch vs the plain path
+    (each member alone, one synchronous reference-structured call per block)."""
+    from kestrel_amd import autotune, workload as W
+    from kestrel_amd.backend import HipBatch, HipSynth
+    warm_s = (args.warmup if args.warmup is not None else 96) * 512 / 48000
+    timed_s = (args.steps if args.steps is not None else 96) * 512 / 48000
+    specs = W.c4_specs(args.seed or 0x4B530004)
+    t = time.time()
+    tuned = {} if args.no_tune else autotune.tune(specs, log=log)
+    log(f"autotune: {len(tuned)} classes in {time.time() - t:.1f} s")
+    members = []
+    for sp in specs:
+        f, w = tuned.get((autotune.size_class(sp["max_voices"]), sp["interpolation"]), (512, 1))
+        cfg, bank, notes = W.c4_render(sp, warm_s + timed_s + 0.1, block_frames=f,
+                                       waves_per_group=w)
+        bb = W.BlockBuilder(cfg, bank, notes)
+        g = HipSynth(cfg, bank.pool, bank.params, bank.n_regions, bank.max_sample_len)
+        members.append(dict(spec=sp, cfg=cfg, bank=bank, bb=bb, g=g,
+                            warm=max(1, int(round(warm_s * 48000 / f))),
+                            timed=max(1, int(round(timed_s * 48000 / f)))))
+    log(f"c4: {len(members)} renders built in {time.time() - t:.1f} s, pools "
+        f"{sum(m['cfg'].max_voices for m in members)} voices in all")
+    batch = HipBatch([m["g"] for m in members])
+    t = time.time()
+    for m in members:
+        stage(m["g"], [m["bb"].next_block() for _ in range(m["warm"])])
+    batch.run(True)
+    log(f"warmup in {time.time() - t:.1f} s, live {sum(m['g'].stats()['active_voices'] for m in members)}")
+    for m in members:
+        m["st0"] = m["g"].stats()
+        m["snap"] = m["g"].pool_export()
+        m["blocks"] = [m["bb"].next_block() for _ in range(m["timed"])]
+        stage(m["g"], m["blocks"])
+    t0 = time.perf_counter()
+    batch.run(True)
+    wall = time.perf_counter() - t0
+    vs_total, audio_total = 0.0, 0.0
+    for m in members:
+        m["outs"], m["stats"] = m["g"].staged_results()
+        V = render_voices(m["cfg"], m["snap"].shape[1], m["blocks"], m["stats"])
+        m["vs"] = float(V.sum()) * m["cfg"].block_frames
+        vs_total += m["vs"]
+        audio_total += m["timed"] * m["cfg"].block_frames / 48000
+    value = vs_total / wall
+    log(f"timed: {len(members)} renders, {audio_total:.1f} s of audio in {wall * 1e3:.1f} ms: "
+        f"{value:.3e} voice-samples/s, {audio_total / wall:.1f}x realtime in aggregate")
+    plain, parity = None, {}
+    if not args.no_plain:
+        tp = 0.0
+        worst = -math.inf
+        equal = True
+        for m in members:
+            pc = m["cfg"].replace(form=0, compensated=False, profile=False)
+            b = m["bank"]
+            p = HipSynth(pc, b.pool, b.params, b.n_regions, b.max_sample_len)
+            p.pool_import(m["snap"])
+            p.set_counters(m["st0"]["stolen"], m["st0"]["dropped"])
+            outs = np.empty_like(m["outs"])
+            t1 = time.perf_counter()
+            for i, blk in enumerate(m["blocks"]):
+                p.set_gates(blk.gates)
+                p.set_channels(blk.chans, *blk.flags)
+                p.spawn(blk.cmds)
+                p.render(outs[i])
+            tp += time.perf_counter() - t1
+            if np.abs(m["outs"]).max() > 0:
+                worst = max(worst, null_db(outs, m["outs"]))
+            equal &= bool(np.array_equal(p.pool_export(), m["g"].pool_export()))
+            p.close()
+        plain = {"value": round(vs_total / tp, 1), "unit": "voice-samples/s",
+                 "x_realtime": round(audio_total / tp, 2),
+                 "form": "each render alone, reference-structured render+reduce, host sync "
+                         "per block"}
+        parity["plain_vs_batch_null_db_worst"] = round(worst, 1)
+        parity["pools_equal_plain_vs_batch"] = equal
+        log(f"plain: {plain}")
+    if not args.no_cpu:
+        from oracle import oracle
+        by_size = sorted(members, key=lambda m: m["snap"].shape[1])
+        checked = []
+        for m in by_size[:2] + by_size[-2:]:  # the two smallest and the two largest
+            b = m["bank"]
+            o = oracle.Oracle(m["cfg"], b.pool, b.params)
+            o.set_sorted(b.n_regions, b.max_sample_len, True)
+            o.pool_import(m["snap"])
+            o.set_counters(m["st0"]["stolen"], m["st0"]["dropped"])
+            blk = m["blocks"][0]
+            o.set_gates(blk.gates)
+            o.set_channels(blk.chans, *blk.flags)
+            o.spawn(blk.cmds)
+            oc = o.render_f64()
+            checked.append(dict(render=m["spec"]["index"], voices=int(m["snap"].shape[1]),
+                                null_db=round(null_db(m["outs"][0], oc), 1)
+                                if np.abs(oc).max() > 0 else None,
+                                stats_match=o.stats()["active_voices"] ==
+                                m["stats"][0]["active_voices"]))
+        parity["cpu_first_block"] = checked
+        log(f"cpu parity: {checked}")
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "voice-samples/s", "n_gpus": 1,
+        "steps": max(m["timed"] for m in members), "warmup": max(m["warm"] for m in members),
+        "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": WORKLOADS["c4"], "name": "c4", "renders": len(members),
+                   "audio_seconds": round(audio_total, 2), "wall_ms": round(wall * 1e3, 2),
+                   "tuned": {f"2^{c}/{'linear' if i == 1 else 'cubic'}": list(v)
+                             for (c, i), v in sorted(tuned.items())}},
+        "x_realtime": round(audio_total / wall, 2),
+        "plain": plain, "vs_plain": round(value / plain["value"], 3) if plain else None,
+        "parity": parity,
+    }
+    batch.close()
+    for m in members:
+        m["g"].close()
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,6 +257,7 @@ def main():
     ap.add_argument("--seed", type=int, default=None)
     ap.add_argument("--waves-per-group", type=int, default=None)
     ap.add_argument("--sort-digit-bits", type=int, default=None)
+    ap.add_argument("--no-tune", action="store_true", help="c4: skip the autotuner (512/1)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -145,6 +268,9 @@ def main():
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.workload == "c4":
+        run_c4(args)
         return
 
     from kestrel_amd.backend import HipSynth

@@ -704,3 +704,41 @@ def workload_c3(seed: int = 0x4B530003, seconds: float = 6.0, cfg: Config | None
                      loop=(16384, 60000), wave="saw", q=(0.5, 20.0), stereo=True)
     notes = burst_notes(seed + 1, seconds, cfg.sample_rate, **note_kw)
     return cfg, bank, notes
+
+
+# ---------------------------------------------------------------------------
+# batching form: many independent renders of uneven size
+# ---------------------------------------------------------------------------
+
+def c4_specs(seed: int = 0x4B530004, n: int = 64) -> list[dict]:
+    """`n` render descriptions: pool 4,096-524,288 voices (log-uniform), note
+    count 1e3-5e6 (log-uniform) over 5-60 s, 4-64 instruments, sample length
+    4k-128k frames, linear or cubic."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        out.append(dict(
+            index=i, seed=seed + 101 * (i + 1),
+            max_voices=int(round(math.exp(rng.uniform(math.log(4096), math.log(524288))))),
+            notes=int(round(math.exp(rng.uniform(math.log(1e3), math.log(5e6))))),
+            seconds=float(rng.uniform(5.0, 60.0)),
+            instruments=int(rng.integers(4, 65)),
+            frames=int(2 ** int(rng.integers(12, 18))),
+            interpolation=int(rng.choice([1, 2]))))
+    return out
+
+
+def c4_render(spec: dict, window_s: float, block_frames: int = 512, waves_per_group: int = 1):
+    """(cfg, bank, notes) for one batch member: its first `window_s` seconds of
+    Poisson note-ons at its own rate, exponential durations (mean 1 s)."""
+    cfg = Config(block_frames=block_frames, max_voices=spec["max_voices"],
+                 interpolation=spec["interpolation"], max_steal_percent=10, compensated=True,
+                 waves_per_group=waves_per_group)
+    frames = spec["frames"]
+    bank = make_bank(spec["seed"], instruments=spec["instruments"], zones=4, key_lo=0,
+                     key_hi=127, frames=frames, loop=(frames // 4, frames - frames // 16))
+    rate = spec["notes"] / spec["seconds"]
+    n = max(1, int(round(rate * window_s)))
+    notes = poisson_notes(spec["seed"] + 1, n, window_s, cfg.sample_rate, key_lo=0, key_hi=127,
+                          mean_dur_s=1.0)
+    return cfg, bank, notes
