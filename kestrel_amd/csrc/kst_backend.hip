@@ -499,6 +499,9 @@ int kst_create(const kst_config *cfg, const int16_t *pool, size_t pool_len,
         b->sort_passes = 1;
     }
     b->tiles_max = (b->slots + SORT_TILE - 1) / SORT_TILE;
+    // A fourth global round keeps the final single-workgroup pass short once
+    // the pool is large enough that three leave tens of thousands of keys.
+    b->sel_rounds = c.max_voices > (1u << 22) ? 4 : 3;
     if (b->fast) {
         b->max_rows = (b->slots + 63) / 64;
         b->max_groups = (b->max_rows + c.waves_per_group - 1) / c.waves_per_group;

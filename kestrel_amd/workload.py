@@ -742,3 +742,88 @@ def c4_render(spec: dict, window_s: float, block_frames: int = 512, waves_per_gr
     notes = poisson_notes(spec["seed"] + 1, n, window_s, cfg.sample_rate, key_lo=0, key_hi=127,
                           mean_dur_s=1.0)
     return cfg, bank, notes
+
+
+# ---------------------------------------------------------------------------
+# extreme pool
+# ---------------------------------------------------------------------------
+
+def config_c5(**kw) -> Config:
+    """Extreme pool: F=512, 67,108,864 voices, steal 5 %, cubic, compensated."""
+    base = dict(block_frames=512, max_voices=1 << 26, interpolation=2, max_steal_percent=5,
+                compensated=True)
+    base.update(kw)
+    return Config(**base)
+
+
+def workload_c5(seed: int = 0x4B530005, seconds: float = 0.25, cfg: Config | None = None,
+                note_rate_hz: float = 4_000_000.0):
+    """128 instruments x 4 zones of stereo 65536-frame samples (each note two
+    voices); Poisson note-ons at `note_rate_hz` plus 50 ms bursts every 0.5 s,
+    exponential durations (mean 1 s), 128 channels (= instruments)."""
+    cfg = cfg or config_c5()
+    bank = make_bank(seed, instruments=128, zones=4, key_lo=0, key_hi=127, frames=65536,
+                     loop=(16384, 60000), stereo=True)
+    rng = np.random.default_rng(seed + 1)
+    n_bg = rng.poisson(note_rate_hz * seconds)
+    t = rng.uniform(0.0, seconds, n_bg)
+    starts = np.arange(0.0, seconds, 0.5)
+    tb = (starts[:, None] + rng.uniform(0.0, 0.05, (len(starts), int(note_rate_hz * 0.05)))).ravel()
+    t = np.concatenate([t, tb[tb < seconds]])
+    n = len(t)
+    notes = _finish(np.floor(t * cfg.sample_rate),
+                    np.maximum(np.round(rng.exponential(1.0, n) * cfg.sample_rate), 1),
+                    rng.integers(0, 128, n), rng.integers(0, 128, n), rng.integers(1, 128, n))
+    return cfg, bank, notes
+
+
+def synthetic_pool(cfg: Config, bank: Bank, n: int, seed: int, chunk: int = 1 << 22) -> np.ndarray:
+    """A full pool in the 24-field SoA layout, as if `n` notes had been playing
+    for a while: random regions and keys, phases inside the sample, stages and
+    levels spread over attack/decay/sustain/release, unique note ids 0..n-1,
+    ordinals that no gate row of a later window reaches."""
+    from .abi import FIELD
+    out = np.zeros((24, n), np.uint32)
+    rng = np.random.default_rng(seed)
+    params = bank.params
+    for lo in range(0, n, chunk):
+        hi = min(n, lo + chunk)
+        m = hi - lo
+        r = rng.integers(0, bank.n_regions, m)
+        key = rng.integers(bank.key_lo[r], bank.key_hi[r] + 1)
+        step = fixed_from_f64(np.exp2((key - bank.root_key[r]) / 12.0))
+        right = (rng.random(m) < 0.5) if bank.stereo else np.zeros(m, bool)
+        base = np.where(right, bank.smp_start_r[r] if bank.stereo else 0, bank.smp_start[r])
+        idx = rng.integers(0, bank.loop_end[r])
+        u = rng.random(m)
+        stage = np.select([u < 0.05, u < 0.15, u < 0.85], [0, 1, 2], 3).astype(np.uint32)
+        sus = params["sustain"][r].astype(np.float32)
+        level = np.select([stage == 0, stage == 1, stage == 2],
+                          [rng.random(m), sus + (1 - sus) * rng.random(m), sus],
+                          sus * rng.random(m) + 1e-4).astype(np.float32)
+        gain = velocity_gain(rng.integers(1, 128, m))
+        gl = np.where(right, 0.0, gain).astype(np.float32) if bank.stereo else gain * F32(0.70710677)
+        gr = np.where(right, gain, 0.0).astype(np.float32) if bank.stereo else gain * F32(0.70710677)
+        ch = rng.integers(0, 16, m)
+        ids = np.arange(lo, hi, dtype=np.uint64)
+        f = out[:, lo:hi]
+        f[FIELD["phase_lo"]] = rng.integers(0, 1 << 32, m, dtype=np.uint64).astype(np.uint32)
+        f[FIELD["phase_hi"]] = idx.astype(np.uint32)
+        f[FIELD["step_lo"]] = (step & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+        f[FIELD["step_hi"]] = (step >> np.uint64(32)).astype(np.uint32)
+        f[FIELD["smp_base"]] = base
+        f[FIELD["smp_len"]] = bank.smp_len[r]
+        f[FIELD["loop_start"]] = bank.loop_start[r]
+        f[FIELD["loop_end"]] = bank.loop_end[r]
+        f[FIELD["flags"]] = bank.loop_flags[r]
+        f[FIELD["env_stage"]] = stage
+        f[FIELD["env_level"]] = level.view(np.uint32)
+        f[FIELD["gain_l"]] = np.asarray(gl, np.float32).view(np.uint32)
+        f[FIELD["gain_r"]] = np.asarray(gr, np.float32).view(np.uint32)
+        f[FIELD["params"]] = r
+        f[FIELD["region"]] = r
+        f[FIELD["gate_slot"]] = (ch * 128 + key).astype(np.uint32)
+        f[FIELD["ordinal"]] = 0x7FFFFFFF
+        f[FIELD["note_lo"]] = (ids & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+        f[FIELD["note_hi"]] = (ids >> np.uint64(32)).astype(np.uint32)
+    return out
