@@ -73,6 +73,17 @@ constexpr uint32_t REDUCE_ROWS = 256;
 // lives here rather than in a kernel argument so a whole block of passes, or a
 // run of blocks, can be launched without the host in between
 // (src/gpu/mod.rs:1-7 made the same move for wgpu).
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup i runs on XCD
+// i % 8). Give each XCD one contiguous run of the sorted pool instead, so its
+// L2 holds the samples of a few regions rather than of all of them. A
+// bijection on [0, n): XCD x takes groups [x*q + min(x, r), ... + q + (x < r)).
+__device__ __forceinline__ uint32_t xcd_group(uint32_t id, uint32_t n) {
+    constexpr uint32_t XCDS = 8;
+    const uint32_t x = id % XCDS, k = id / XCDS;
+    const uint32_t q = n / XCDS, r = n % XCDS;
+    return x * q + (x < r ? x : r) + k;
+}
+
 struct DevState {
     uint32_t live;    // compacted live count at the start of the block
     uint32_t parity;  // which pool buffer holds the live pool

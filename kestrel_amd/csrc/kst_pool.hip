@@ -658,8 +658,14 @@ __device__ __forceinline__ void gather_body(const Args &a) {
     const uint32_t *src = sel2(a.pools, st->parity);
     uint32_t *dst = sel2(a.pools, st->parity ^ 1);
     const uint32_t c = a.cap;
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n;
-         j += gridDim.x * blockDim.x) {
+    // Each workgroup copies one contiguous run of the sorted order, and each
+    // XCD a contiguous set of runs: the source slots a run reads are the
+    // neighbourhood the previous block's order left them in, so they share
+    // that XCD's L2.
+    const uint32_t per = (n + gridDim.x - 1) / gridDim.x;
+    const uint32_t j0 = xcd_group(blockIdx.x, gridDim.x) * per;
+    const uint32_t j1 = j0 + per < n ? j0 + per : n;
+    for (uint32_t j = j0 + threadIdx.x; j < j1; j += blockDim.x) {
         const bool alive = key_alive(a, skey[j]);
         const bool next_alive = j + 1 < n && key_alive(a, skey[j + 1]);
         if (alive && !next_alive) st->live_new = j + 1;

@@ -659,17 +659,6 @@ __device__ __forceinline__ void fast_subtile_edge(const Args &a, Voice &v, float
     }
 }
 
-// Workgroups are dealt to the 8 XCDs round-robin (workgroup i runs on XCD
-// i % 8). Give each XCD one contiguous run of the sorted pool instead, so its
-// L2 holds the samples of a few regions rather than of all of them. A
-// bijection on [0, n): XCD x takes groups [x*q + min(x, r), ... + q + (x < r)).
-__device__ __forceinline__ uint32_t xcd_group(uint32_t id, uint32_t n) {
-    constexpr uint32_t XCDS = 8;
-    const uint32_t x = id % XCDS, k = id / XCDS;
-    const uint32_t q = n / XCDS, r = n % XCDS;
-    return x * q + (x < r ? x : r) + k;
-}
-
 // Error-free transformation of a + b (Knuth's two-sum).
 __device__ __forceinline__ void two_sum(float a, float b, float &s, float &e) {
     s = a + b;
@@ -841,23 +830,29 @@ __device__ __forceinline__ void reduce_final_body(const Args &a) {
     DevState *st = a.st;
     if (st->idle) return;
     const uint32_t two_f = 2 * a.F;
-    const uint32_t col = blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= two_f) return;
+    // FQ threads per output sample, each over a contiguous quarter of the
+    // chunks; the quarters are joined in order.
+    constexpr uint32_t FQ = 4;
+    __shared__ float qh[FQ][64], ql[FQ][64];
+    const uint32_t q = threadIdx.x / 64, t = threadIdx.x % 64;
+    const uint32_t col = blockIdx.x * 64 + t;
     const uint32_t rows = (st->live_total + 63) / 64;
     const uint32_t chunks = (rows + REDUCE_ROWS - 1) / REDUCE_ROWS;
+    const uint32_t cq = (chunks + FQ - 1) / FQ;
+    const uint32_t cbeg = q * cq, cend = cbeg + cq < chunks ? cbeg + cq : chunks;
     float h = 0.0f, l = 0.0f;
     constexpr uint32_t CU = 8;
-    for (uint32_t c0 = 0; c0 < chunks; c0 += CU) {
+    for (uint32_t c0 = cbeg; c0 < cend && col < two_f; c0 += CU) {
         float2 x[CU];
 #pragma unroll
         for (uint32_t u = 0; u < CU; ++u) {
             const uint32_t c = c0 + u;
-            x[u] = c < chunks ? reinterpret_cast<const float2 *>(a.rsum)[(size_t)c * two_f + col]
-                              : make_float2(0.0f, 0.0f);
+            x[u] = c < cend ? reinterpret_cast<const float2 *>(a.rsum)[(size_t)c * two_f + col]
+                            : make_float2(0.0f, 0.0f);
         }
 #pragma unroll
         for (uint32_t u = 0; u < CU; ++u) {
-            if (c0 + u >= chunks) break;
+            if (c0 + u >= cend) break;
             if (COMP) {
                 float s, e;
                 two_sum(h, x[u].x, s, e);
@@ -866,6 +861,20 @@ __device__ __forceinline__ void reduce_final_body(const Args &a) {
             } else {
                 h = h + x[u].x;
             }
+        }
+    }
+    qh[q][t] = h;
+    ql[q][t] = l;
+    __syncthreads();
+    if (q != 0 || col >= two_f) return;
+    for (uint32_t k = 1; k < FQ; ++k) {
+        if (COMP) {
+            float s, e;
+            two_sum(h, qh[k][t], s, e);
+            h = s;
+            l = l + (e + ql[k][t]);
+        } else {
+            h = h + qh[k][t];
         }
     }
     const float v = COMP ? h + l : h;
@@ -1035,7 +1044,7 @@ void launch_batch_render(const Args *aa, uint32_t nr, uint32_t interp, bool chan
 void launch_batch_reduce(const Args *aa, uint32_t nr, bool comp, uint32_t max_two_f,
                          uint32_t max_rows, hipStream_t s) {
     const dim3 g1((max_two_f + 63) / 64, (max_rows + REDUCE_ROWS - 1) / REDUCE_ROWS, nr);
-    const dim3 g2((max_two_f + 255) / 256, 1, nr);
+    const dim3 g2((max_two_f + 63) / 64, 1, nr);
     if (comp) {
         kb_reduce_rows<true><<<g1, 256, 0, s>>>(aa);
         kb_reduce_final<true><<<g2, 256, 0, s>>>(aa);
@@ -1048,7 +1057,7 @@ void launch_batch_reduce(const Args *aa, uint32_t nr, bool comp, uint32_t max_tw
 void launch_reduce_fast(const Args &a, bool comp, uint32_t max_rows, hipStream_t s) {
     const uint32_t two_f = 2 * a.F;
     const dim3 g1((two_f + 63) / 64, (max_rows + REDUCE_ROWS - 1) / REDUCE_ROWS);
-    const uint32_t g2 = (two_f + 255) / 256;
+    const uint32_t g2 = (two_f + 63) / 64;
     if (comp) {
         k_reduce_rows<true><<<g1, 256, 0, s>>>(a);
         k_reduce_final<true><<<g2, 256, 0, s>>>(a);
