@@ -61,6 +61,30 @@ def stage(backend, blocks):
                          np.array([len(b.cmds) for b in blocks], np.uint32))
 
 
+def wav_null(dev_blocks, cpu_blocks, sample_rate) -> dict:
+    """The three null figures: raw mix absolute (dBFS), and what `kestrel null`
+    reports on the two renders after the reference's default post-render step
+    (true-peak brickwall, then the hard clamp), each side through its own
+    limiter, block by block."""
+    from kestrel_amd import post
+    dev = np.asarray(dev_blocks, np.float32).reshape(len(dev_blocks), -1)
+    cpu = np.asarray(cpu_blocks, np.float64).astype(np.float32).reshape(len(cpu_blocks), -1)
+    raw = float(np.max(np.abs(dev.astype(np.float64) - np.asarray(cpu_blocks, np.float64))))
+    la = post.Limiter(sample_rate=sample_rate, mode=post.BRICKWALL)
+    lb = post.Limiter(sample_rate=sample_rate, mode=post.BRICKWALL)
+    a, b = cpu.copy(), dev.copy()
+    for i in range(len(a)):
+        la.process(a[i])
+        post.clamp_block(a[i])
+        lb.process(b[i])
+        post.clamp_block(b[i])
+    r = post.null(a.ravel(), b.ravel())
+    return {"raw_null_dbfs": round(20 * math.log10(raw), 1) if raw > 0 else None,
+            "wav_null_dbfs": round(r["peak_diff_db"], 1) if r["peak_diff"] > 0 else None,
+            "wav_null_rms_dbfs": round(r["rms_diff_db"], 1) if r["rms_diff"] > 0 else None,
+            "wav_identical": r["identical"]}
+
+
 def render_voices(cfg, live0, blocks, stats):
     """Voices entering render per block: live + spawned (src/gpu/mod.rs:1015-1040)."""
     out = []
@@ -474,6 +498,7 @@ def main():
                                   f"({tcpu:.1f} s, single-threaded restatement of src/cpu.rs)",
                         "x_realtime": round(nb * F / cfg.sample_rate / tcpu, 4)}
         parity["null_db_vs_cpu"] = round(null_db(outs[:nb], oc), 1)
+        parity.update(wav_null(outs[:nb], oc, cfg.sample_rate))
         parity["blocks_checked"] = nb
         parity["stats_match_cpu"] = o.stats()["active_voices"] == stats[nb - 1]["active_voices"]
         if nb > 1:

@@ -200,6 +200,55 @@ typedef struct kst_region_design {
  * kst_set_params_variant. */
 int kst_design_params(kst_backend *b, uint32_t index, const kst_region_design *in, size_t n);
 
+/* ---- post-render step: the caller's side of render() ----
+ *
+ * What the reference's driver does with every rendered block (src/driver.rs:
+ * 596-607): the limiter, then the hard clamp; the WAV writer of src/wav.rs and
+ * the comparison of `kestrel null` (src/bin/kestrel.rs:597-646). Host code: a
+ * block is a sequential recurrence over 2 * F samples. */
+#define KST_LIMITER_OFF 0
+#define KST_LIMITER_OMNI 1      /* Limiter, src/limiter.rs:14-118 */
+#define KST_LIMITER_BRICKWALL 2 /* Brickwall, src/limiter.rs:203-446 */
+typedef struct kst_limiter_config {
+    uint32_t sample_rate;
+    uint32_t mode;       /* KST_LIMITER_* (LimiterMode, src/limiter.rs:134-149) */
+    double ceiling_db;   /* brickwall ceiling, dBFS */
+    double lookahead_ms; /* brickwall lookahead (also its latency) */
+    double release_ms;   /* brickwall fast-stage release */
+    double sustain_ms;   /* brickwall sustained stage; 0 disables it */
+    uint32_t true_peak;  /* 4x oversampled peak detection */
+    uint32_t pad;
+} kst_limiter_config;
+typedef struct kst_limiter kst_limiter;
+/* src/config.rs:388-395 */
+void kst_default_limiter_config(kst_limiter_config *cfg);
+int kst_limiter_create(const kst_limiter_config *cfg, kst_limiter **out);
+/* Limiter::process / Brickwall::process on one interleaved stereo block, in place. */
+int kst_limiter_process(kst_limiter *l, float *buf, size_t n);
+/* Brickwall::latency, frames (0 for the other modes). */
+size_t kst_limiter_latency(const kst_limiter *l);
+void kst_limiter_destroy(kst_limiter *l);
+/* clamp_block, src/limiter.rs:120-129: returns how many samples it moved. */
+uint64_t kst_clamp_block(float *buf, size_t n);
+
+/* WavWriter, src/wav.rs:50-159: float32 != 0 writes IEEE float, else PCM16. */
+typedef struct kst_wav kst_wav;
+int kst_wav_create(const char *path, uint32_t sample_rate, uint32_t channels, uint32_t float32,
+                   kst_wav **out);
+int kst_wav_write(kst_wav *w, const float *samples, size_t n);
+/* Patches the sizes (saturating at 4 GiB), closes and frees. */
+int kst_wav_finish(kst_wav *w);
+
+/* `kestrel null` over the first min(na, nb) samples; `a` is the reference. */
+typedef struct kst_null_result {
+    uint64_t compared, worst;
+    double peak_ref, peak_diff, rms_diff;
+    uint32_t identical, pad;
+} kst_null_result;
+int kst_null(const float *a, size_t na, const float *b, size_t nb, kst_null_result *out);
+/* The message of the last post-render failure on this thread. */
+const char *kst_post_last_error(void);
+
 /* ---- test and checkpoint hooks (no counterpart in the reference) ---- */
 
 /* Copy params variant `index` back to the host (n == params per variant). */

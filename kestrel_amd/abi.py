@@ -84,6 +84,17 @@ SIGNATURES = {
     "kst_batch_run": (i32, [P, i32]),
     "kst_batch_destroy": (None, [P]),
     "kst_design_params": (i32, [P, u32, P, sz]),
+    "kst_default_limiter_config": (None, [P]),
+    "kst_limiter_create": (i32, [P, P]),
+    "kst_limiter_process": (i32, [P, P, sz]),
+    "kst_limiter_latency": (sz, [P]),
+    "kst_limiter_destroy": (None, [P]),
+    "kst_clamp_block": (u64, [P, sz]),
+    "kst_wav_create": (i32, [C.c_char_p, u32, u32, u32, P]),
+    "kst_wav_write": (i32, [P, P, sz]),
+    "kst_wav_finish": (i32, [P]),
+    "kst_null": (i32, [P, sz, P, sz, P]),
+    "kst_post_last_error": (C.c_char_p, []),
     "kst_params_export": (i32, [P, u32, P, sz]),
     "kst_pool_export": (i64, [P, P, sz]),
     "kst_pool_import": (i32, [P, P, sz, sz]),

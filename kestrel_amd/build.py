@@ -21,7 +21,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB_NAME = "libkestrel_hip.so"
 LIB_PATH = os.path.join(HERE, LIB_NAME)
-SOURCES = ["kst_backend.hip", "kst_pool.hip", "kst_render.hip", "kst_design.hip"]
+SOURCES = ["kst_backend.hip", "kst_pool.hip", "kst_render.hip", "kst_design.hip", "kst_post.cpp"]
 ARCH = os.environ.get("KST_OFFLOAD_ARCH", "gfx950")
 
 
@@ -50,7 +50,7 @@ def build(verbose: bool = False) -> str:
     objs = []
     procs = []
     for src in SOURCES:
-        obj = os.path.join(objdir, src.replace(".hip", ".o"))
+        obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
         cmd = [cc, *flags(), "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
