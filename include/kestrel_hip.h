@@ -249,6 +249,155 @@ int kst_null(const float *a, size_t na, const float *b, size_t nb, kst_null_resu
 /* The message of the last post-render failure on this thread. */
 const char *kst_post_last_error(void);
 
+/* ---- the caller of the backend: the driver's block loop ----
+ *
+ * src/driver.rs:240-1192 with what it stands on: the flattened instrument bank
+ * (src/bank.rs), the note-off gate and controller tables (src/voice.rs:212-658)
+ * and the streaming SMF reader with its tempo clock (src/midi.rs). Host code.
+ * Bank, MIDI and driver failures leave their message for kst_host_last_error(). */
+
+/* SampleInfo, src/bank.rs:35-54 (the name is not kept). 32 bytes. */
+typedef struct kst_sample_info {
+    uint32_t start, len;             /* frames in the bank's pool */
+    uint32_t loop_start, loop_end;   /* relative to start */
+    uint32_t rate;                   /* Hz the data is stored at */
+    float correction_cents;
+    float resample_ratio;            /* pool frames per source frame */
+    uint32_t root_key;
+} kst_sample_info;
+
+#define KST_LOOP_NONE 0
+#define KST_LOOP_CONTINUOUS 1
+#define KST_LOOP_UNTIL_RELEASE 2
+/* Region, src/bank.rs:57-120, every generator resolved. 96 bytes. */
+typedef struct kst_region {
+    uint32_t sample;
+    uint8_t key_lo, key_hi, vel_lo, vel_hi;
+    int16_t root_key_override, fixed_key, fixed_vel, coarse_tune;  /* -1: unset */
+    int16_t fine_tune, scale_tuning, keynum_to_hold, keynum_to_decay;
+    float attenuation_cb, amp_veltrack, pan;
+    uint32_t loop_mode;              /* KST_LOOP_* */
+    int32_t addr_start, addr_end, addr_loop_start, addr_loop_end;
+    uint32_t exclusive_class;
+    float delay, attack, hold, decay, sustain, release;  /* seconds; sustain a level */
+    float filter_fc_cents, filter_q_cb, filter_veltrack_cents;
+} kst_region;
+
+/* Preset, src/bank.rs:352-362: its regions are preset_regions[first .. first+count]. */
+typedef struct kst_preset {
+    uint32_t bank, program, first, count;
+} kst_preset;
+
+/* The driver-side settings of src/config.rs that kst_config does not carry. */
+typedef struct kst_driver_config {
+    uint32_t max_layers;             /* :207 */
+    float master_volume;             /* :260 */
+    uint32_t default_channel_volume; /* :266 */
+    uint32_t filter_enabled;         /* :258 */
+    uint32_t limiter;                /* :273, run limiter_cfg.mode on each block */
+    uint32_t nan_guard;              /* :328, fail on a non-finite output sample */
+    kst_limiter_config limiter_cfg;  /* :280-313 */
+} kst_driver_config;
+/* src/config.rs:345-409 */
+void kst_default_driver_config(kst_driver_config *cfg);
+
+typedef struct kst_bank kst_bank;
+/* Bank::finish + Bank::build_params (src/bank.rs:514-527, 689-781): lays out the
+ * params table, designs variant 0 and the admission preview tables. Uses the
+ * sample_rate, env_floor and curves of `cfg`. pool_rate 0 means mixed-rate. */
+int kst_bank_create(const kst_config *cfg, const kst_driver_config *dcfg, const int16_t *pool,
+                    size_t pool_len, uint32_t pool_rate, const kst_sample_info *samples,
+                    size_t n_samples, const kst_region *regions, size_t n_regions,
+                    const kst_preset *presets, size_t n_presets, const uint32_t *preset_regions,
+                    size_t n_preset_regions, kst_bank **out);
+/* What kst_create takes from the bank (src/gpu/mod.rs:346-388). */
+int kst_bank_info(const kst_bank *b, size_t *pool_len, size_t *n_params, uint32_t *n_regions,
+                  uint32_t *max_sample_len);
+const int16_t *kst_bank_pool(const kst_bank *b);
+const kst_region_params *kst_bank_params(const kst_bank *b);
+/* Bank::build_variant (src/bank.rs:836-853) for ParamMod::from_controllers of
+ * cc[0..5] = CC71..CC75 (src/bank.rs:336-350). n == params per variant. */
+int kst_bank_variant(const kst_bank *b, const uint8_t cc[5], kst_region_params *out, size_t n);
+/* Bank::find_preset, src/bank.rs:529-546; -1 when the bank has no presets. */
+int64_t kst_bank_find_preset(const kst_bank *b, uint32_t bank, uint32_t program);
+void kst_bank_destroy(kst_bank *b);
+
+/* Division, src/midi.rs:32-36: smpte == 0 uses ppq, else fps * ticks_per_frame. */
+typedef struct kst_division {
+    uint32_t smpte, ppq, fps, ticks_per_frame;
+} kst_division;
+#define KST_EV_NOTE_ON 0    /* a = key, b = velocity */
+#define KST_EV_NOTE_OFF 1   /* a = key */
+#define KST_EV_CC 2         /* a = controller, b = value */
+#define KST_EV_PROGRAM 3    /* a = program */
+#define KST_EV_PITCH_BEND 4 /* value = -8192..8191 */
+#define KST_EV_TEMPO 5      /* value = microseconds per quarter note */
+#define KST_EV_OTHER 6
+/* (tick, Event), src/midi.rs:17-29. 24 bytes. */
+typedef struct kst_midi_event {
+    uint64_t tick;
+    uint32_t kind;                   /* KST_EV_* */
+    uint8_t ch, a, b, pad;
+    int32_t value;
+    uint32_t pad2;
+} kst_midi_event;
+
+typedef struct kst_midi kst_midi;
+/* MidiStream::open, src/midi.rs:259-345: a tick-ordered merge of every track,
+ * read through a 256 KiB window per track. */
+int kst_midi_open(const char *path, kst_midi **out);
+int kst_midi_info(const kst_midi *m, kst_division *div, uint32_t *format, uint32_t *tracks);
+/* MidiStream::next, src/midi.rs:351-361, up to `cap` at a time; returns how
+ * many were written, 0 at the end of the file. */
+int64_t kst_midi_read(kst_midi *m, kst_midi_event *out, size_t cap);
+void kst_midi_close(kst_midi *m);
+
+/* DriverStats, src/driver.rs:143-186. */
+typedef struct kst_driver_stats {
+    uint64_t variant_fallbacks, variant_rebuilds, frames, blocks, events, dropped, notes,
+        voices_spawned, clipped, last_stolen, last_want_energy, last_take_energy;
+    uint32_t param_variants, variant_states, last_live, last_want, last_take;
+    float peak;
+} kst_driver_stats;
+
+typedef struct kst_driver kst_driver;
+/* Driver::open, src/driver.rs:343-413. The bank must outlive the driver. */
+int kst_driver_open(const kst_config *cfg, const kst_driver_config *dcfg, const kst_bank *bank,
+                    const char *midi_path, kst_driver **out);
+/* The same over an already merged, tick-ordered event list (copied). */
+int kst_driver_create(const kst_config *cfg, const kst_driver_config *dcfg,
+                      const kst_bank *bank, const kst_division *div,
+                      const kst_midi_event *events, size_t n, kst_driver **out);
+/* Driver::next_block, src/driver.rs:434-642, against this library's backend:
+ * returns 1 while the file or its tail has more, 0 when done, < 0 on failure.
+ * `n` == block_frames * 2. */
+int kst_driver_next_block(kst_driver *d, kst_backend *b, float *out, size_t n);
+/* The same block in the steps a caller with another backend drives itself:
+ * begin (events, rows, variants; returns how many variants to upload), then
+ * the rows and variants for that backend, admit (given its live voices and
+ * stolen counter before spawn), spawn and render there, then end (limiter,
+ * clamp, counters; returns 1 while there is more). */
+int kst_driver_begin_block(kst_driver *d);
+int kst_driver_pending_variant(const kst_driver *d, size_t i, uint32_t *index,
+                               const kst_region_params **data, size_t *n);
+int kst_driver_rows(const kst_driver *d, const uint32_t **gates, size_t *n_gates,
+                    const uint32_t **chans, size_t *n_chans, uint32_t flags[3]);
+int kst_driver_admit(kst_driver *d, uint64_t live, uint64_t stolen_before,
+                     const kst_spawn_cmd **cmds, size_t *n);
+int kst_driver_end_block(kst_driver *d, float *out, size_t n, uint64_t stolen_after_spawn,
+                         const kst_block_stats *after_render);
+int kst_driver_get_stats(const kst_driver *d, kst_driver_stats *out);
+void kst_driver_destroy(kst_driver *d);
+
+#define KST_CC_APPLIED 0
+#define KST_CC_INERT 1
+#define KST_CC_MISSING 2
+/* cc_role, src/driver.rs:58-131: returns KST_CC_*, with the controller's name
+ * and what the role means. */
+int kst_cc_role(uint32_t num, const char **name, const char **note);
+/* The message of the last bank, MIDI or driver failure on this thread. */
+const char *kst_host_last_error(void);
+
 /* ---- test and checkpoint hooks (no counterpart in the reference) ---- */
 
 /* Copy params variant `index` back to the host (n == params per variant). */

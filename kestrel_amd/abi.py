@@ -44,6 +44,39 @@ DESIGN_DTYPE = np.dtype([
 ])
 assert DESIGN_DTYPE.itemsize == 64
 
+# kst_sample_info, src/bank.rs:35-54 (32 B)
+SAMPLE_DTYPE = np.dtype([
+    ("start", "<u4"), ("len", "<u4"), ("loop_start", "<u4"), ("loop_end", "<u4"),
+    ("rate", "<u4"), ("correction_cents", "<f4"), ("resample_ratio", "<f4"), ("root_key", "<u4"),
+])
+assert SAMPLE_DTYPE.itemsize == 32
+
+# kst_region, src/bank.rs:57-120 (96 B)
+REGION_DTYPE = np.dtype([
+    ("sample", "<u4"), ("key_lo", "u1"), ("key_hi", "u1"), ("vel_lo", "u1"), ("vel_hi", "u1"),
+    ("root_key_override", "<i2"), ("fixed_key", "<i2"), ("fixed_vel", "<i2"),
+    ("coarse_tune", "<i2"), ("fine_tune", "<i2"), ("scale_tuning", "<i2"),
+    ("keynum_to_hold", "<i2"), ("keynum_to_decay", "<i2"),
+    ("attenuation_cb", "<f4"), ("amp_veltrack", "<f4"), ("pan", "<f4"), ("loop_mode", "<u4"),
+    ("addr_start", "<i4"), ("addr_end", "<i4"), ("addr_loop_start", "<i4"),
+    ("addr_loop_end", "<i4"), ("exclusive_class", "<u4"),
+    ("delay", "<f4"), ("attack", "<f4"), ("hold", "<f4"), ("decay", "<f4"), ("sustain", "<f4"),
+    ("release", "<f4"), ("filter_fc_cents", "<f4"), ("filter_q_cb", "<f4"),
+    ("filter_veltrack_cents", "<f4"),
+])
+assert REGION_DTYPE.itemsize == 96
+
+# kst_preset, src/bank.rs:352-362 (16 B)
+PRESET_DTYPE = np.dtype([("bank", "<u4"), ("program", "<u4"), ("first", "<u4"), ("count", "<u4")])
+
+# kst_midi_event: (tick, Event), src/midi.rs:17-29 (24 B)
+EVENT_DTYPE = np.dtype([
+    ("tick", "<u8"), ("kind", "<u4"), ("ch", "u1"), ("a", "u1"), ("b", "u1"), ("pad", "u1"),
+    ("value", "<i4"), ("pad2", "<u4"),
+])
+assert EVENT_DTYPE.itemsize == 24
+EV_NOTE_ON, EV_NOTE_OFF, EV_CC, EV_PROGRAM, EV_PITCH_BEND, EV_TEMPO, EV_OTHER = range(7)
+
 VOICE_FIELDS = 24
 # shaders/common.wgsl:15-43
 FIELDS = ("phase_lo", "phase_hi", "step_lo", "step_hi", "smp_base", "smp_len", "loop_start",
@@ -99,6 +132,30 @@ SIGNATURES = {
     "kst_pool_export": (i64, [P, P, sz]),
     "kst_pool_import": (i32, [P, P, sz, sz]),
     "kst_set_counters": (i32, [P, u64, u64]),
+    "kst_default_driver_config": (None, [P]),
+    "kst_bank_create": (i32, [P, P, P, sz, u32, P, sz, P, sz, P, sz, P, sz, P]),
+    "kst_bank_info": (i32, [P, P, P, P, P]),
+    "kst_bank_pool": (P, [P]),
+    "kst_bank_params": (P, [P]),
+    "kst_bank_variant": (i32, [P, P, P, sz]),
+    "kst_bank_find_preset": (i64, [P, u32, u32]),
+    "kst_bank_destroy": (None, [P]),
+    "kst_midi_open": (i32, [C.c_char_p, P]),
+    "kst_midi_info": (i32, [P, P, P, P]),
+    "kst_midi_read": (i64, [P, P, sz]),
+    "kst_midi_close": (None, [P]),
+    "kst_driver_open": (i32, [P, P, P, C.c_char_p, P]),
+    "kst_driver_create": (i32, [P, P, P, P, P, sz, P]),
+    "kst_driver_next_block": (i32, [P, P, P, sz]),
+    "kst_driver_begin_block": (i32, [P]),
+    "kst_driver_pending_variant": (i32, [P, sz, P, P, P]),
+    "kst_driver_rows": (i32, [P, P, P, P, P, P]),
+    "kst_driver_admit": (i32, [P, u64, u64, P, P]),
+    "kst_driver_end_block": (i32, [P, P, sz, u64, P]),
+    "kst_driver_get_stats": (i32, [P, P]),
+    "kst_driver_destroy": (None, [P]),
+    "kst_cc_role": (i32, [u32, P, P]),
+    "kst_host_last_error": (C.c_char_p, []),
 }
 
 _lib = None

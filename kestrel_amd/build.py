@@ -21,7 +21,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB_NAME = "libkestrel_hip.so"
 LIB_PATH = os.path.join(HERE, LIB_NAME)
-SOURCES = ["kst_backend.hip", "kst_pool.hip", "kst_render.hip", "kst_design.hip", "kst_post.cpp"]
+SOURCES = ["kst_backend.hip", "kst_pool.hip", "kst_render.hip", "kst_design.hip", "kst_post.cpp",
+           "kst_bank.cpp", "kst_midi.cpp", "kst_driver.cpp"]
 ARCH = os.environ.get("KST_OFFLOAD_ARCH", "gfx950")
 
 

@@ -48,6 +48,18 @@ class Config:
     max_param_variants: int = 32
     # ---- numerics (src/config.rs:324-334) ----
     kahan_reduce: bool = False
+    nan_guard: bool = False
+    # ---- driver and output (src/config.rs:207, 258-313) ----
+    max_layers: int = 16
+    master_volume: float = 1.0
+    default_channel_volume: int = 127
+    limiter: bool = True
+    limiter_mode: int = 2  # 0 Off, 1 Omni, 2 Brickwall
+    limiter_ceiling_db: float = 0.0
+    limiter_lookahead_ms: float = 2.0
+    limiter_release_ms: float = 60.0
+    limiter_sustain_ms: float = 0.0
+    limiter_true_peak: bool = True
     # ---- MI355X knobs (no counterpart in the reference) ----
     form: int = FORM_WAVE64
     compensated: bool = False

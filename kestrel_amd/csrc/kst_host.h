@@ -1,0 +1,191 @@
+// kst_host.h -- host-side classes of the driver's block loop: the flattened
+// instrument bank, the SMF event stream and tempo clock, and the driver.
+// The C ABI over them is in include/kestrel_hip.h; nothing here touches the GPU.
+#pragma once
+
+#include <cstdint>
+#include <cstdio>
+#include <memory>
+#include <queue>
+#include <string>
+#include <vector>
+
+#include "kestrel_hip.h"
+
+namespace kst {
+
+// ---------------------------------------------------------------------------
+// Saturating conversions with Rust's `as` semantics (NaN -> 0, clamp to range),
+// which several reference formulas rely on.
+// ---------------------------------------------------------------------------
+inline uint32_t sat_u32(float v) {
+    if (!(v > 0.0f)) return 0;
+    if (v >= 4294967296.0f) return UINT32_MAX;
+    return (uint32_t)v;
+}
+inline uint64_t sat_u64(double v) {
+    if (!(v > 0.0)) return 0;
+    if (v >= 18446744073709551616.0) return UINT64_MAX;
+    return (uint64_t)v;
+}
+inline int64_t sat_i64(float v) {
+    if (v != v) return 0;
+    if (v >= 9223372036854775808.0f) return INT64_MAX;
+    if (v < -9223372036854775808.0f) return INT64_MIN;
+    return (int64_t)v;
+}
+
+// ---------------------------------------------------------------------------
+// Bank: src/bank.rs. Regions arrive with every generator resolved; the bank
+// lays out the params table, designs variants and answers note-on previews.
+// ---------------------------------------------------------------------------
+struct ParamMod {  // src/bank.rs:206-217
+    float cutoff_cents = 0.0f, q_cb = 0.0f;
+    float attack_scale = 1.0f, attack_add = 0.0f;
+    float decay_scale = 1.0f, decay_add = 0.0f;
+    float release_scale = 1.0f, release_add = 0.0f;
+    static ParamMod from_controllers(uint8_t cc71, uint8_t cc72, uint8_t cc73, uint8_t cc74,
+                                     uint8_t cc75);
+};
+
+struct Preview {  // PreviewLayer, src/bank.rs:421-431
+    uint32_t region;
+    float gain_l, gain_r;
+    uint32_t delay_frames;
+};
+
+struct Layer {  // VoiceSpawn, src/bank.rs:433-450
+    uint64_t phase, step;
+    uint32_t smp_base, smp_len, loop_start, loop_end, flags, params, region;
+    float gain_l, gain_r;
+    uint32_t delay_frames;
+    uint32_t exclusive_class;
+};
+
+struct BankSettings {
+    uint32_t sample_rate = 48000, decay_curve = 0, release_curve = 0, filter_enabled = 1;
+    float env_floor = 1.0e-5f, master_volume = 1.0f;
+};
+
+class Bank {
+public:
+    std::vector<int16_t> pool;
+    uint32_t pool_rate = 0;
+    std::vector<kst_sample_info> samples;
+    std::vector<kst_region> regions;
+    BankSettings set;
+    // Per region: params layout (src/bank.rs:689-721).
+    std::vector<uint32_t> params_base, params_stride, params_vel_span;
+    std::vector<kst_region_params> params;  // variant 0
+    uint32_t max_sample_len = 0;
+
+    // Builds the layout, variant 0 and the admission tables. Throws
+    // std::invalid_argument naming what is wrong with the input.
+    Bank(const BankSettings &s, const int16_t *pool, size_t pool_len, uint32_t pool_rate,
+         const kst_sample_info *samples, size_t n_samples, const kst_region *regions,
+         size_t n_regions, const kst_preset *presets, size_t n_presets,
+         const uint32_t *preset_regions, size_t n_preset_regions);
+
+    int64_t find_preset(uint32_t bank, uint32_t program) const;  // -1: no presets
+    void preview_note_on(uint32_t preset, uint8_t key, uint8_t vel, size_t max_layers,
+                         std::vector<Preview> &out) const;
+    bool build_layer(uint32_t region, uint8_t key, uint8_t vel, Layer &out) const;
+    void build_variant(const ParamMod &m, std::vector<kst_region_params> &out) const;
+
+private:
+    struct PresetEntry {
+        uint32_t bank, program;
+        std::vector<uint32_t> key_index;    // 129 offsets into key_regions
+        std::vector<uint32_t> key_regions;
+    };
+    std::vector<PresetEntry> presets_;
+    std::vector<std::pair<uint64_t, uint32_t>> index_;  // ((bank<<16)|program, preset), sorted
+    std::vector<float> gain_table_;                     // region*128+vel -> (l, r)
+    std::vector<uint32_t> delay_frames_;
+    std::vector<uint64_t> key_ok_;
+    bool key_ok(uint32_t region, uint8_t key) const {
+        const size_t bit = (size_t)region * 128 + key;
+        return (key_ok_[bit / 64] >> (bit % 64)) & 1;
+    }
+    bool build_voice(const kst_region &r, uint32_t ri, uint8_t key, uint8_t vel, Layer &out) const;
+    kst_region_params make_params(const kst_region &r, uint8_t key, uint8_t vel,
+                                  const ParamMod &m) const;
+};
+
+// ---------------------------------------------------------------------------
+// Event sources: the streaming SMF reader (src/midi.rs:38-358) or a list.
+// ---------------------------------------------------------------------------
+class EventSource {
+public:
+    virtual ~EventSource() = default;
+    // Next event in tick order; false at the end.
+    virtual bool next(kst_midi_event &ev) = 0;
+    kst_division division{};
+};
+
+class MidiFile : public EventSource {
+public:
+    // Throws std::runtime_error with the reference's message on a bad file.
+    explicit MidiFile(const std::string &path);
+    ~MidiFile() override;
+    bool next(kst_midi_event &ev) override;
+    uint32_t format = 0, track_count = 0;
+
+private:
+    struct Track {
+        int fd = -1;
+        uint64_t off = 0;  // file offset of the next unread chunk byte
+        std::vector<uint8_t> buf;
+        size_t pos = 0, filled = 0;
+        uint64_t remaining = 0, tick = 0;
+        uint8_t running = 0;
+        bool ended = false;
+        bool fill();
+        bool byte(uint8_t &b);
+        bool varlen(uint64_t &v);
+        bool skip(uint64_t n);
+        bool next_event(kst_midi_event &ev);
+    };
+    int fd_ = -1;
+    std::vector<Track> tracks_;
+    std::vector<kst_midi_event> pending_;
+    std::priority_queue<std::pair<uint64_t, uint32_t>, std::vector<std::pair<uint64_t, uint32_t>>,
+                        std::greater<std::pair<uint64_t, uint32_t>>>
+        heap_;
+};
+
+class EventList : public EventSource {
+public:
+    EventList(const kst_midi_event *ev, size_t n, const kst_division &div)
+        : events_(ev, ev + n) {
+        division = div;
+    }
+    bool next(kst_midi_event &ev) override {
+        if (at_ >= events_.size()) return false;
+        ev = events_[at_++];
+        return true;
+    }
+
+private:
+    std::vector<kst_midi_event> events_;
+    size_t at_ = 0;
+};
+
+// src/midi.rs:363-418
+class TempoClock {
+public:
+    TempoClock(const kst_division &d, uint32_t sample_rate);
+    void set_tempo(uint64_t tick, uint32_t us_per_qn);
+    double frame_at(uint64_t tick) const {
+        return base_frame_ + (double)(tick - base_tick_) * frames_per_tick_;
+    }
+
+private:
+    void set_raw(uint32_t us_per_qn);
+    kst_division div_;
+    double sample_rate_;
+    uint64_t base_tick_ = 0;
+    double base_frame_ = 0.0, frames_per_tick_ = 0.0;
+};
+
+}  // namespace kst
