@@ -110,6 +110,11 @@ WORKLOADS = {
           "buffer) started full from a synthetic snapshot, steal-percent 5, cubic, 128x4 stereo "
           "65536-frame regions, 4M notes/s Poisson with 50 ms bursts, exponential 1 s durations, "
           "compensated mixdown",
+    "midi": "end to end: a dense format-1 SMF (16 tracks, 2-60 ms notes, 4M note-ons/s by "
+            "default) read by the native driver (src/driver.rs block loop: events, gate and "
+            "controller rows, ranked admission, voice build) into the 1,048,576-voice backend "
+            "(512-frame blocks, steal-percent 10, linear, compensated), then the brickwall "
+            "limiter and clamp, one synchronous block at a time as the reference runs",
     "c3": "precision form: 48 kHz stereo, 512-frame blocks, 2,097,152-voice pool (steal Oldest "
           "10 %), stereo 65536-frame saw+noise samples as two hard-panned voices, cubic "
           "interpolation, +-2 semitone bend random walks, lowpass Q log-uniform 0.5-20 designed "
@@ -269,6 +274,114 @@ def run_c4(args):
     print(json.dumps(line), flush=True)
 
 
+def run_midi(args):
+    """MIDI file -> native driver -> backend -> limiter, block by block. Reports
+    the end-to-end rate, the driver's own host cost per block (the same file
+    through a second driver with a saturated pool stand-in, no device calls),
+    and the oracle's Python driver on the first blocks as the CPU baseline."""
+    import tempfile
+    from kestrel_amd import workload as W
+    from kestrel_amd.backend import HipSynth
+    from kestrel_amd.driver import Bank, Driver
+    K = args.steps if args.steps is not None else 200
+    WU = args.warmup if args.warmup is not None else 200
+    rate = args.notes_per_s
+    cfg = W.config_c2(compensated=True, limiter=True, limiter_mode=2)
+    F = cfg.block_frames
+    seconds = (WU + K + 2) * F / cfg.sample_rate
+    seed = args.seed or 0x4B530006
+    t = time.time()
+    desc = W.driver_bank_desc(seed)
+    fd, path = tempfile.mkstemp(suffix=".mid")
+    os.close(fd)
+    size = W.dense_midi(path, seed + 1, int(rate * seconds), seconds)
+    bank = Bank(cfg, *desc)
+    log(f"midi: {int(rate * seconds)} notes over {seconds:.2f} s, {size / 1e6:.1f} MB SMF, "
+        f"bank {bank.n_regions} regions, built in {time.time() - t:.1f} s")
+    gpu = HipSynth(cfg, bank.pool, bank.params, bank.n_regions, bank.max_sample_len)
+    drv = Driver.open(cfg, bank, path)
+    out = np.zeros(F * 2, np.float32)
+    for _ in range(WU):
+        drv.next_block(gpu, out)
+    s0 = drv.stats()
+    t0 = time.perf_counter()
+    for _ in range(K):
+        drv.next_block(gpu, out)
+    wall = time.perf_counter() - t0
+    s1 = drv.stats()
+    live = gpu.stats()["active_voices"]
+    notes = s1["notes"] - s0["notes"]
+    x_rt = K * F / cfg.sample_rate / wall
+    phases = {k: round(v, 4) for k, v in drv.timings().items()}
+    log(f"timed: {K} blocks in {wall * 1e3:.1f} ms ({wall / K * 1e3:.3f} ms/block), {x_rt:.2f}x "
+        f"realtime, {notes / wall:.3e} note-ons/s, live {live}, dropped "
+        f"{s1['dropped'] - s0['dropped']}, last take {s1['last_take']}; host ms/block by phase "
+        f"(warm-up included) {phases}")
+    gpu.close()
+
+    # The driver alone: same file, the pool held at capacity, its outputs
+    # left in place (no copies to Python).
+    import ctypes as C
+    from kestrel_amd import abi
+    lib = abi.load()
+    host = Driver.open(cfg, bank, path)
+    zero = np.zeros(F * 2, np.float32)
+    st = dict(active_voices=cfg.max_voices, stolen=0, dropped=0, peak=0.0)
+    kst = abi.KstBlockStats(cfg.max_voices, 0, 0, 0.0, 0)
+    cp, cn = C.c_void_p(), C.c_size_t()
+
+    def host_block():
+        lib.kst_driver_begin_block(host._h)
+        lib.kst_driver_admit(host._h, cfg.max_voices, 0, C.byref(cp), C.byref(cn))
+        lib.kst_driver_end_block(host._h, abi.ptr(zero), zero.size, 0, C.byref(kst))
+    for _ in range(WU):
+        host_block()
+    th = time.perf_counter()
+    for _ in range(K):
+        host_block()
+    host_s = (time.perf_counter() - th) / K
+    log(f"driver host cost: {host_s * 1e3:.3f} ms/block ({host_s / (F / cfg.sample_rate):.2f} of "
+        f"a block's duration)")
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import driver as OD
+        settings = dict(sample_rate=cfg.sample_rate, decay_curve=cfg.decay_curve,
+                        release_curve=cfg.release_curve, filter_enabled=cfg.filter_enabled,
+                        env_floor=np.float32(cfg.env_floor), master_volume=cfg.master_volume)
+        ob = OD.Bank(settings, *desc[1:], pool_rate=0)
+        div, oev = OD.read_midi(path)
+        od = OD.Driver(cfg, ob, div, oev)
+        nb, tc = 0, time.perf_counter()
+        while nb < 3 or time.perf_counter() - tc < 10.0:
+            od.begin_block()
+            od.admit(cfg.max_voices, 0)
+            od.end_block(zero.copy(), 0, st)
+            nb += 1
+            if nb >= 40:
+                break
+        tcpu = (time.perf_counter() - tc) / nb
+        cpu = {"value": round(od.stats["notes"] / (tcpu * nb), 1), "unit": "note-ons/s",
+               "cores": 1, "kind": "port",
+               "sample": f"the oracle's Python driver over the first {nb} blocks of the same file "
+                         f"(host only, pool at capacity), {tcpu * 1e3:.1f} ms/block"}
+        log(f"cpu baseline: {cpu}")
+    os.unlink(path)
+    line = {
+        "metric": "x-realtime end to end (MIDI file -> driver -> 1M-voice render -> limiter)",
+        "value": round(x_rt, 3), "unit": "x realtime", "n_gpus": 1, "steps": K, "warmup": WU,
+        "ms_per_step": round(wall / K * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": WORKLOADS["midi"], "name": "midi", "block_frames": F,
+                   "max_voices": cfg.max_voices, "notes_per_s": rate, "live_voices": live},
+        "note_ons_per_s": round(notes / wall, 1),
+        "driver_host_ms_per_block": round(host_s * 1e3, 4),
+        "driver_phases_ms_per_block": phases,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def run_c5(args):
     """The extreme pool: throughput over the timed blocks, then the first timed
     block again from the same snapshot against the CPU oracle in state-only
@@ -370,6 +483,7 @@ def main():
     ap.add_argument("--waves-per-group", type=int, default=None)
     ap.add_argument("--sort-digit-bits", type=int, default=None)
     ap.add_argument("--no-tune", action="store_true", help="c4: skip the autotuner (512/1)")
+    ap.add_argument("--notes-per-s", type=float, default=4e6, help="midi: note-on rate")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -386,6 +500,9 @@ def main():
         return
     if args.workload == "c5":
         run_c5(args)
+        return
+    if args.workload == "midi":
+        run_midi(args)
         return
 
     from kestrel_amd.backend import HipSynth

@@ -129,6 +129,14 @@ int kst_spawn(kst_backend *b, const kst_spawn_cmd *cmds, size_t n);
  * stereo. Advances every voice and compacts the pool; synchronous. */
 int kst_render(kst_backend *b, float *out, size_t n);
 
+/* The two halves of kst_render (no counterpart in the reference): submit
+ * queues the block's uploads, passes and readback on the backend's stream and
+ * returns at once; collect waits and writes `out`. In between the caller may
+ * work on the host (kst_stats still answers), but every other call on this
+ * backend fails with KST_ESTATE until the render is collected. */
+int kst_render_submit(kst_backend *b);
+int kst_render_collect(kst_backend *b, float *out, size_t n);
+
 /* Backend::stats -- src/backend.rs:49. */
 int kst_stats(const kst_backend *b, kst_block_stats *out);
 
@@ -370,7 +378,10 @@ int kst_driver_create(const kst_config *cfg, const kst_driver_config *dcfg,
                       const kst_midi_event *events, size_t n, kst_driver **out);
 /* Driver::next_block, src/driver.rs:434-642, against this library's backend:
  * returns 1 while the file or its tail has more, 0 when done, < 0 on failure.
- * `n` == block_frames * 2. */
+ * `n` == block_frames * 2. While the device renders a block, the host already
+ * drains the events of the next one and builds its rows (nothing of that
+ * depends on the render); what each call returns and what kst_driver_get_stats
+ * reports after it are those of the reference's sequential loop. */
 int kst_driver_next_block(kst_driver *d, kst_backend *b, float *out, size_t n);
 /* The same block in the steps a caller with another backend drives itself:
  * begin (events, rows, variants; returns how many variants to upload), then
@@ -387,6 +398,11 @@ int kst_driver_admit(kst_driver *d, uint64_t live, uint64_t stolen_before,
 int kst_driver_end_block(kst_driver *d, float *out, size_t n, uint64_t stolen_after_spawn,
                          const kst_block_stats *after_render);
 int kst_driver_get_stats(const kst_driver *d, kst_driver_stats *out);
+/* Host time per block of kst_driver_next_block, by phase (events and rows,
+ * admission and voice build, post-render step, the backend calls, the wait
+ * for the device); writes up to `cap` entries and returns how many there are.
+ * No counterpart in the reference. */
+size_t kst_driver_timings(const kst_driver *d, kst_timing *out, size_t cap);
 void kst_driver_destroy(kst_driver *d);
 
 #define KST_CC_APPLIED 0

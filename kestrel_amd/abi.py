@@ -106,6 +106,8 @@ SIGNATURES = {
     "kst_set_params_variant": (i32, [P, u32, P, sz]),
     "kst_spawn": (i32, [P, P, sz]),
     "kst_render": (i32, [P, P, sz]),
+    "kst_render_submit": (i32, [P]),
+    "kst_render_collect": (i32, [P, P, sz]),
     "kst_stats": (i32, [P, P]),
     "kst_name": (C.c_char_p, [P]),
     "kst_timings": (sz, [P, P, sz]),
@@ -153,6 +155,7 @@ SIGNATURES = {
     "kst_driver_admit": (i32, [P, u64, u64, P, P]),
     "kst_driver_end_block": (i32, [P, P, sz, u64, P]),
     "kst_driver_get_stats": (i32, [P, P]),
+    "kst_driver_timings": (sz, [P, P, sz]),
     "kst_driver_destroy": (None, [P]),
     "kst_cc_role": (i32, [u32, P, P]),
     "kst_host_last_error": (C.c_char_p, []),

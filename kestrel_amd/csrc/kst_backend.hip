@@ -208,6 +208,7 @@ struct kst_backend {
     float peak = 0.0f;
 
     bool profile = false;
+    bool inflight = false;  // a render submitted and not yet collected
     // Select rounds launched per block. A round whose bin is already small
     // enough returns at once; rounds still needed after these are finished by
     // the final single-workgroup pass, so the count only affects speed.
@@ -604,6 +605,7 @@ void kst_destroy(kst_backend *b) {
 }
 
 int kst_set_gates(kst_backend *b, const uint32_t *rows, size_t n) {
+    if (b && b->inflight) return fail(KST_ESTATE, "a submitted render has not been collected");
     if (!b || (!rows && n)) return fail(KST_EINVAL, "null argument");
     if (n != b->gate_words)
         return fail(KST_EINVAL, "gate rows are %zu words, expected %u", n, b->gate_words);
@@ -613,6 +615,7 @@ int kst_set_gates(kst_backend *b, const uint32_t *rows, size_t n) {
 
 int kst_set_channels(kst_backend *b, const uint32_t *rows, size_t n, int bend, int gain,
                      int variant) {
+    if (b && b->inflight) return fail(KST_ESTATE, "a submitted render has not been collected");
     if (!b || (!rows && n)) return fail(KST_EINVAL, "null argument");
     if (n != b->chan_words)
         return fail(KST_EINVAL, "channel rows are %zu words, expected %u", n, b->chan_words);
@@ -624,6 +627,7 @@ int kst_set_channels(kst_backend *b, const uint32_t *rows, size_t n, int bend, i
 
 int kst_set_params_variant(kst_backend *b, uint32_t index, const kst_region_params *data,
                            size_t n) {
+    if (b && b->inflight) return fail(KST_ESTATE, "a submitted render has not been collected");
     if (!b || (!data && n)) return fail(KST_EINVAL, "null argument");
     // src/gpu/mod.rs:972-987
     if (n != b->ppv)
@@ -637,6 +641,7 @@ int kst_set_params_variant(kst_backend *b, uint32_t index, const kst_region_para
 }
 
 int kst_design_params(kst_backend *b, uint32_t index, const kst_region_design *in, size_t n) {
+    if (b && b->inflight) return fail(KST_ESTATE, "a submitted render has not been collected");
     if (!b || (!in && n)) return fail(KST_EINVAL, "null argument");
     if (n != b->ppv)
         return fail(KST_EINVAL, "params variant %u has %zu entries, expected %u", index, n, b->ppv);
@@ -672,6 +677,7 @@ int kst_params_export(kst_backend *b, uint32_t index, kst_region_params *out, si
 }
 
 int kst_spawn(kst_backend *b, const kst_spawn_cmd *cmds, size_t n) {
+    if (b && b->inflight) return fail(KST_ESTATE, "a submitted render has not been collected");
     if (!b || (!cmds && n)) return fail(KST_EINVAL, "null argument");
     if (n > 0xFFFFFFFFull) return fail(KST_EINVAL, "%zu spawn commands in one block", n);
     b->pending.assign(cmds, cmds + n);
@@ -688,10 +694,10 @@ int kst_spawn(kst_backend *b, const kst_spawn_cmd *cmds, size_t n) {
     return KST_OK;
 }
 
-int kst_render(kst_backend *b, float *out, size_t n) {
-    if (!b || !out) return fail(KST_EINVAL, "null argument");
+int kst_render_submit(kst_backend *b) {
+    if (!b) return fail(KST_EINVAL, "null argument");
+    if (b->inflight) return fail(KST_ESTATE, "a submitted render has not been collected");
     const size_t two_f = 2 * (size_t)b->cfg.block_frames;
-    if (n != two_f) return fail(KST_EINVAL, "output block is %zu samples, expected %zu", n, two_f);
     HIPCHK(hipSetDevice(b->device));
     const uint32_t ncmd = (uint32_t)b->pending.size();
     int rc = b->ensure_stage(1, ncmd);
@@ -737,7 +743,18 @@ int kst_render(kst_backend *b, float *out, size_t n) {
     HIPCHK(hipMemcpyAsync(b->pin_out, b->g_out.p, two_f * 4, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(b->pin_result, b->g_results.p, sizeof(BlockResult),
                           hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
+    b->inflight = true;
+    return KST_OK;
+}
+
+int kst_render_collect(kst_backend *b, float *out, size_t n) {
+    if (!b || !out) return fail(KST_EINVAL, "null argument");
+    const size_t two_f = 2 * (size_t)b->cfg.block_frames;
+    if (n != two_f) return fail(KST_EINVAL, "output block is %zu samples, expected %zu", n, two_f);
+    if (!b->inflight) return fail(KST_ESTATE, "no render was submitted");
+    HIPCHK(hipSetDevice(b->device));
+    b->inflight = false;
+    HIPCHK(hipStreamSynchronize(b->stream));
     std::memcpy(out, b->pin_out, two_f * 4);
     const BlockResult &r = *b->pin_result;
     b->live = (uint32_t)r.active_voices;
@@ -753,6 +770,14 @@ int kst_render(kst_backend *b, float *out, size_t n) {
     if (b->profile) b->read_timings(b->ev, 1);
     b->pending.clear();
     return KST_OK;
+}
+
+int kst_render(kst_backend *b, float *out, size_t n) {
+    if (!b || !out) return fail(KST_EINVAL, "null argument");
+    const size_t two_f = 2 * (size_t)b->cfg.block_frames;
+    if (n != two_f) return fail(KST_EINVAL, "output block is %zu samples, expected %zu", n, two_f);
+    if (int rc = kst_render_submit(b)) return rc;
+    return kst_render_collect(b, out, n);
 }
 
 int kst_stats(const kst_backend *b, kst_block_stats *out) {
@@ -791,6 +816,7 @@ size_t kst_timings(const kst_backend *b, kst_timing *out, size_t cap) {
 int kst_stage_blocks(kst_backend *b, uint32_t nblocks, const uint32_t *gates,
                      const uint32_t *chans, const uint32_t *chan_flags,
                      const kst_spawn_cmd *cmds, const uint32_t *cmd_counts) {
+    if (b && b->inflight) return fail(KST_ESTATE, "a submitted render has not been collected");
     if (!b || !gates || !chans || !chan_flags || !cmd_counts || nblocks == 0)
         return fail(KST_EINVAL, "null argument or zero blocks");
     HIPCHK(hipSetDevice(b->device));
@@ -823,6 +849,7 @@ int kst_stage_blocks(kst_backend *b, uint32_t nblocks, const uint32_t *gates,
 }
 
 int kst_run_staged(kst_backend *b, int use_graph) {
+    if (b && b->inflight) return fail(KST_ESTATE, "a submitted render has not been collected");
     if (!b) return fail(KST_EINVAL, "null argument");
     if (!b->staged) return fail(KST_ESTATE, "no staged blocks");
     HIPCHK(hipSetDevice(b->device));
@@ -1084,6 +1111,7 @@ int kst_batch_run(kst_batch *bt, int use_graph) {
 }
 
 int64_t kst_pool_export(kst_backend *b, uint32_t *soa, size_t stride) {
+    if (b && b->inflight) return fail(KST_ESTATE, "a submitted render has not been collected");
     if (!b || !soa) return fail(KST_EINVAL, "null argument");
     if (stride < b->live) return fail(KST_EINVAL, "stride %zu below the live count %u", stride, b->live);
     if (hipSetDevice(b->device) != hipSuccess) return fail(KST_EDEVICE, "hipSetDevice");
@@ -1097,6 +1125,7 @@ int64_t kst_pool_export(kst_backend *b, uint32_t *soa, size_t stride) {
 }
 
 int kst_pool_import(kst_backend *b, const uint32_t *soa, size_t n, size_t stride) {
+    if (b && b->inflight) return fail(KST_ESTATE, "a submitted render has not been collected");
     if (!b || (!soa && n)) return fail(KST_EINVAL, "null argument");
     if (n > b->cfg.max_voices)
         return fail(KST_EINVAL, "%zu voices exceed max_voices %u", n, b->cfg.max_voices);
@@ -1115,6 +1144,7 @@ int kst_pool_import(kst_backend *b, const uint32_t *soa, size_t n, size_t stride
 }
 
 int kst_set_counters(kst_backend *b, uint64_t stolen, uint64_t dropped) {
+    if (b && b->inflight) return fail(KST_ESTATE, "a submitted render has not been collected");
     if (!b) return fail(KST_EINVAL, "null argument");
     HIPCHK(hipSetDevice(b->device));
     DevState s;

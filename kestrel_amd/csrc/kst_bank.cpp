@@ -197,6 +197,24 @@ Bank::Bank(const BankSettings &s, const int16_t *pool_in, size_t pool_len, uint3
             }
         }
     }
+    // Layer build tables: every (region, key) that builds, through the same
+    // build_voice, so a lookup returns exactly what the call would.
+    plan_.assign(n_regions, RegionPlan{});
+    step_.assign(n_regions * 128, 0);
+    for (size_t ri = 0; ri < n_regions; ++ri) {
+        bool first = true;
+        for (uint32_t k = 0; k < 128; ++k) {
+            if (!key_ok((uint32_t)ri, (uint8_t)k)) continue;
+            Layer v;
+            build_voice(regions[ri], (uint32_t)ri, (uint8_t)k, 64, v);
+            step_[ri * 128 + k] = v.step;
+            if (first) {
+                plan_[ri] = RegionPlan{(uint32_t)(v.phase >> 32), v.smp_len, v.loop_start,
+                                       v.loop_end, v.flags};
+                first = false;
+            }
+        }
+    }
 }
 
 int64_t Bank::find_preset(uint32_t bank, uint32_t program) const {  // src/bank.rs:529-546
@@ -233,9 +251,32 @@ void Bank::preview_note_on(uint32_t preset, uint8_t key, uint8_t vel, size_t max
     }
 }
 
-bool Bank::build_layer(uint32_t region, uint8_t key, uint8_t vel, Layer &out) const {
+bool Bank::build_layer(uint32_t region, uint8_t key, uint8_t vel, Layer &o) const {
     if (region >= regions.size()) return false;  // src/bank.rs:585-594
-    return build_voice(regions[region], region, std::min<uint8_t>(key, 127), vel, out);
+    key = std::min<uint8_t>(key, 127);
+    if (vel > 127) return build_voice(regions[region], region, key, vel, o);
+    if (!key_ok(region, key)) return false;  // exactly where build_voice gives up
+    const kst_region &r = regions[region];
+    const RegionPlan &p = plan_[region];
+    const uint8_t eff_key = r.fixed_key >= 0 ? (uint8_t)r.fixed_key : key;
+    const uint32_t span = params_vel_span[region];
+    const uint32_t vi = span > 1 ? std::min<uint32_t>(vel > r.vel_lo ? vel - r.vel_lo : 0, span - 1) : 0;
+    const size_t g = ((size_t)region * 128 + vel) * 2;
+    o.phase = (uint64_t)p.start_offset << 32;
+    o.step = step_[(size_t)region * 128 + key];
+    o.smp_base = samples[r.sample].start;
+    o.smp_len = p.smp_len;
+    o.loop_start = p.loop_start;
+    o.loop_end = p.loop_end;
+    o.flags = p.flags;
+    o.params = params_base[region] + vi * (params_stride[region] ? 128u : 1u) +
+               (params_stride[region] ? (uint32_t)eff_key : 0u);
+    o.region = region;
+    o.gain_l = gain_table_[g];
+    o.gain_r = gain_table_[g + 1];
+    o.delay_frames = delay_frames_[region];
+    o.exclusive_class = r.exclusive_class;
+    return true;
 }
 
 bool Bank::build_voice(const kst_region &r, uint32_t ri, uint8_t key, uint8_t vel,

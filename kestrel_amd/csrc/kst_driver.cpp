@@ -11,6 +11,7 @@
 // the limiter, the hard clamp, and the end-of-file tail.
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -321,8 +322,17 @@ public:
     // Everything before admission (src/driver.rs:450-555). Returns how many
     // params variants wait to be uploaded.
     size_t begin_block() {
+        if (ahead_) {  // already begun while the device rendered the last block
+            take_ahead_stats();
+            return uploads_.size();
+        }
+        return begin_at(block_index_);
+    }
+
+    size_t begin_at(uint64_t index) {
+        const auto t0 = Clock::now();
         const uint64_t F = cfg_.block_frames;
-        const uint64_t block_start = block_index_ * F, block_end = block_start + F;
+        const uint64_t block_start = index * F, block_end = block_start + F;
         gates_.begin_block();
         chan_.begin_block();
         variant_used_ = 1;
@@ -396,11 +406,13 @@ public:
         chan_.end_block();
         apply_modulation();
         chan_.refresh_active();
+        t_events_ += Clock::now() - t0;
         return uploads_.size();
     }
 
     // Admission and the build of what it admits (src/driver.rs:556-594).
     void admit(uint64_t live64, uint64_t stolen_before) {
+        const auto t0 = Clock::now();
         const uint32_t live = (uint32_t)live64;
         const size_t want = cands_.size();
         const size_t take = admit_take(cfg_, live, (uint32_t)want);
@@ -424,10 +436,24 @@ public:
         }
         materialise(std::min(take, cands_.size()));
         stats_.voices_spawned += want;
+        t_admit_ += Clock::now() - t0;
     }
 
     // After the render (src/driver.rs:596-641). Returns whether there is more.
     int end_block(float *out, size_t n, uint64_t stolen_after_spawn, const kst_block_stats &st) {
+        return end_at(out, n, stolen_after_spawn, st, !stream_done_ || !deferred_.empty());
+    }
+
+    // `src_more`: the file or the deferred queue still had something after
+    // this block's own events were drained.
+    int end_at(float *out, size_t n, uint64_t stolen_after_spawn, const kst_block_stats &st,
+               bool src_more) {
+        const auto t0 = Clock::now();
+        struct Tally {
+            Clock::time_point t0;
+            Clock::duration &acc;
+            ~Tally() { acc += Clock::now() - t0; }
+        } tally{t0, t_post_};
         stats_.last_stolen = stolen_after_spawn - stolen_before_;
         if (limiter_ && kst_limiter_process(limiter_, out, n) != KST_OK)
             return fail(KST_EINVAL, kst_post_last_error());
@@ -442,7 +468,7 @@ public:
         stats_.frames += cfg_.block_frames;
         stats_.blocks += 1;
         ++block_index_;
-        if (!stream_done_ || !deferred_.empty()) return 1;
+        if (src_more) return 1;
         if (st.active_voices > 0) {
             tail_blocks_left_ = tail_blocks_left_ > 0 ? tail_blocks_left_ - 1 : 0;
             return tail_blocks_left_ == 0 ? 0 : 1;
@@ -450,6 +476,10 @@ public:
         return 0;
     }
 
+    // One block against this library's backend. The device renders block b
+    // while the host drains the events of b+1 and builds its rows: none of
+    // that reads the render, and its counters are held back so that what the
+    // caller sees after each call is what the sequential loop would show.
     int next_block(kst_backend *b, float *out, size_t n) {
         if (n != (size_t)cfg_.block_frames * 2)
             return fail(KST_EINVAL, "output block is " + std::to_string(n) + " samples, expected " +
@@ -459,6 +489,7 @@ public:
             if (rc != KST_OK) throw std::runtime_error(kst_last_error());
         };
         try {
+            auto tb = Clock::now();
             for (const auto &u : uploads_)
                 be(kst_set_params_variant(b, u.first, u.second.data(), u.second.size()));
             be(kst_set_gates(b, gates_.rows.data(), gates_.rows.size()));
@@ -466,16 +497,42 @@ public:
                                 chan_.gain_active, chan_.variant_active || spawn_variants_));
             kst_block_stats st{};
             be(kst_stats(b, &st));
+            t_backend_ += Clock::now() - tb;
             admit(st.active_voices, st.stolen);
+            tb = Clock::now();
             be(kst_spawn(b, spawn_buf_.data(), spawn_buf_.size()));
             be(kst_stats(b, &st));
             const uint64_t stolen_after = st.stolen;
-            be(kst_render(b, out, n));
+            be(kst_render_submit(b));
+            t_backend_ += Clock::now() - tb;
+            const bool src_more = !stream_done_ || !deferred_.empty();
+            const kst_driver_stats published = stats_;
+            begin_at(block_index_ + 1);
+            ahead_stats_ = stats_;
+            stats_ = published;
+            ahead_ = true;
+            tb = Clock::now();
+            be(kst_render_collect(b, out, n));
             be(kst_stats(b, &st));
-            return end_block(out, n, stolen_after, st);
+            t_wait_ += Clock::now() - tb;
+            ++timed_blocks_;
+            return end_at(out, n, stolen_after, st, src_more);
         } catch (const std::runtime_error &e) {
             return fail(KST_EDEVICE, e.what());
         }
+    }
+
+    size_t timings(kst_timing *out, size_t cap) const {
+        if (timed_blocks_ == 0) return 0;
+        const double k = 1e-6 / (double)timed_blocks_;
+        const kst_timing t[5] = {
+            {"events_rows", std::chrono::duration<double, std::nano>(t_events_).count() * k},
+            {"admission_build", std::chrono::duration<double, std::nano>(t_admit_).count() * k},
+            {"post_render", std::chrono::duration<double, std::nano>(t_post_).count() * k},
+            {"backend_calls", std::chrono::duration<double, std::nano>(t_backend_).count() * k},
+            {"device_wait", std::chrono::duration<double, std::nano>(t_wait_).count() * k}};
+        for (size_t i = 0; i < 5 && i < cap; ++i) out[i] = t[i];
+        return 5;
     }
 
     const std::vector<std::pair<uint32_t, std::vector<kst_region_params>>> &uploads() const {
@@ -542,6 +599,24 @@ private:
     uint64_t block_first_id_ = 0;
     std::vector<std::pair<uint64_t, kst_spawn_cmd>> deferred_;
     kst_driver_stats stats_{};
+    // The next block, begun early by next_block, and its counters.
+    bool ahead_ = false;
+    kst_driver_stats ahead_stats_{};
+    typedef std::chrono::steady_clock Clock;
+    Clock::duration t_events_{}, t_admit_{}, t_post_{}, t_backend_{}, t_wait_{};
+    uint64_t timed_blocks_ = 0;
+
+    // Counters begin_at moved for the early block: events, notes and the
+    // variant bookkeeping. Nothing after it in a block touches those.
+    void take_ahead_stats() {
+        stats_.events = ahead_stats_.events;
+        stats_.notes = ahead_stats_.notes;
+        stats_.variant_states = ahead_stats_.variant_states;
+        stats_.param_variants = ahead_stats_.param_variants;
+        stats_.variant_rebuilds = ahead_stats_.variant_rebuilds;
+        stats_.variant_fallbacks = ahead_stats_.variant_fallbacks;
+        ahead_ = false;
+    }
 
     static State neutral_state() {
         State s;
@@ -1022,7 +1097,7 @@ int kst_driver_open(const kst_config *cfg, const kst_driver_config *dcfg, const 
     if (int rc = check_driver_config(cfg, dcfg)) return rc;
     std::unique_ptr<kst::EventSource> src;
     try {
-        src = std::make_unique<kst::MidiFile>(midi_path);
+        src = std::make_unique<kst::Prefetch>(std::make_unique<kst::MidiFile>(midi_path));
     } catch (const std::exception &e) {
         return fail(KST_EINVAL, e.what());
     }
@@ -1096,6 +1171,11 @@ int kst_driver_get_stats(const kst_driver *d, kst_driver_stats *out) {
     if (!d || !out) return fail(KST_EINVAL, "null argument");
     *out = d->d->stats();
     return KST_OK;
+}
+
+size_t kst_driver_timings(const kst_driver *d, kst_timing *out, size_t cap) {
+    if (!d || (!out && cap)) return 0;
+    return d->d->timings(out, cap);
 }
 
 void kst_driver_destroy(kst_driver *d) { delete d; }

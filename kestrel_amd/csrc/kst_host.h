@@ -3,11 +3,15 @@
 // The C ABI over them is in include/kestrel_hip.h; nothing here touches the GPU.
 #pragma once
 
+#include <condition_variable>
 #include <cstdint>
 #include <cstdio>
+#include <deque>
 #include <memory>
+#include <mutex>
 #include <queue>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "kestrel_hip.h"
@@ -103,6 +107,13 @@ private:
     std::vector<float> gain_table_;                     // region*128+vel -> (l, r)
     std::vector<uint32_t> delay_frames_;
     std::vector<uint64_t> key_ok_;
+    // What build_voice computes from the region alone, and the phase step of
+    // each (region, key): a layer build is then table lookups.
+    struct RegionPlan {
+        uint32_t start_offset, smp_len, loop_start, loop_end, flags;
+    };
+    std::vector<RegionPlan> plan_;
+    std::vector<uint64_t> step_;                        // region*128+key
     bool key_ok(uint32_t region, uint8_t key) const {
         const size_t bit = (size_t)region * 128 + key;
         return (key_ok_[bit / 64] >> (bit % 64)) & 1;
@@ -145,13 +156,30 @@ private:
         bool varlen(uint64_t &v);
         bool skip(uint64_t n);
         bool next_event(kst_midi_event &ev);
+        // Decoded events not yet merged, [at, n) of ev.
+        std::vector<kst_midi_event> ev;
+        size_t at = 0, n = 0;
+        bool refill();
     };
     int fd_ = -1;
     std::vector<Track> tracks_;
-    std::vector<kst_midi_event> pending_;
-    std::priority_queue<std::pair<uint64_t, uint32_t>, std::vector<std::pair<uint64_t, uint32_t>>,
-                        std::greater<std::pair<uint64_t, uint32_t>>>
-        heap_;
+    // Winner tree over the tracks' head events keyed (tick << 20 | track), so
+    // one integer minimum orders by tick and then by track; an exhausted
+    // track's key is all ones. node_[1] is the next event; leaves sit at
+    // [leaves_, 2 * leaves_). A tick too wide for the key (2^44 and up)
+    // switches the merge to a scan over (tick, track) pairs.
+    uint32_t leaves_ = 1;
+    std::vector<uint64_t> node_;
+    bool wide_ = false;
+    // A run: track run_w_ keeps winning while its head key stays below the
+    // best of the other tracks, run_limit_ -- dense files give runs of many
+    // events at one tick, emitted without touching the tree.
+    bool run_ = false;
+    uint32_t run_w_ = 0;
+    uint64_t run_limit_ = 0;
+    uint64_t leaf_key(uint32_t t) const;
+    void update(uint32_t t);
+    uint32_t scan_min() const;
 };
 
 class EventList : public EventSource {
@@ -169,6 +197,28 @@ public:
 private:
     std::vector<kst_midi_event> events_;
     size_t at_ = 0;
+};
+
+// Decodes another source ahead on a thread of its own, in chunks through a
+// bounded queue: the consumer sees exactly the inner sequence, and the file's
+// decoding leaves the block loop's critical path.
+class Prefetch : public EventSource {
+public:
+    explicit Prefetch(std::unique_ptr<EventSource> inner, size_t chunk = 1 << 15, size_t depth = 32);
+    ~Prefetch() override;
+    bool next(kst_midi_event &ev) override;
+
+private:
+    void run();
+    std::unique_ptr<EventSource> inner_;
+    size_t chunk_, depth_;
+    std::mutex m_;
+    std::condition_variable cv_;
+    std::deque<std::vector<kst_midi_event>> q_;
+    bool done_ = false, stop_ = false;
+    std::vector<kst_midi_event> cur_;
+    size_t at_ = 0;
+    std::thread th_;
 };
 
 // src/midi.rs:363-418

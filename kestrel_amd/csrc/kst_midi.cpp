@@ -93,30 +93,92 @@ MidiFile::MidiFile(const std::string &path) {
     const size_t window =
         std::max(MIN_WINDOW, std::min(TRACK_WINDOW, WINDOW_BUDGET / locs.size()));
     fd_ = fd;
-    tracks_.resize(locs.size());
-    pending_.resize(locs.size());
-    for (size_t i = 0; i < locs.size(); ++i) {
+    const uint32_t T = (uint32_t)locs.size();
+    wide_ = T >= (1u << 20);
+    tracks_.resize(T);
+    for (uint32_t i = 0; i < T; ++i) {
         Track &t = tracks_[i];
         t.fd = fd;
         t.off = locs[i].first;
         t.remaining = locs[i].second;
         t.buf.resize((size_t)std::min<uint64_t>(window, std::max<uint64_t>(locs[i].second, 1)));
-        if (t.next_event(pending_[i])) heap_.emplace(t.tick, (uint32_t)i);
+        t.refill();
     }
+    while (leaves_ < T) leaves_ <<= 1;
+    node_.assign(2 * (size_t)leaves_, UINT64_MAX);
+    for (uint32_t i = 0; i < T; ++i) node_[leaves_ + i] = leaf_key(i);
+    for (uint32_t i = leaves_ - 1; i >= 1; --i) node_[i] = std::min(node_[2 * i], node_[2 * i + 1]);
 }
 
 MidiFile::~MidiFile() {
     if (fd_ >= 0) close(fd_);
 }
 
+uint64_t MidiFile::leaf_key(uint32_t t) const {
+    const Track &k = tracks_[t];
+    if (k.at == k.n) return UINT64_MAX;
+    const uint64_t tick = k.ev[k.at].tick;
+    if (tick >= (1ull << 44)) return UINT64_MAX - 1;  // marks the wide case
+    return (tick << 20) | t;
+}
+
+// The next event by (tick, track) over every track, for ticks too wide for
+// the packed key (src/midi.rs:246-249).
+uint32_t MidiFile::scan_min() const {
+    uint32_t best = UINT32_MAX;
+    for (uint32_t t = 0; t < tracks_.size(); ++t) {
+        const Track &k = tracks_[t];
+        if (k.at == k.n) continue;
+        if (best == UINT32_MAX || k.ev[k.at].tick < tracks_[best].ev[tracks_[best].at].tick) best = t;
+    }
+    return best;
+}
+
+// Decode the next batch of a track's events (its tick carried in each).
+bool MidiFile::Track::refill() {
+    if (ev.empty()) ev.resize(256);
+    at = n = 0;
+    while (n < ev.size() && next_event(ev[n])) ev[n++].tick = tick;
+    return n > 0;
+}
+
+void MidiFile::update(uint32_t t) {
+    uint32_t i = leaves_ + t;
+    node_[i] = leaf_key(t);
+    for (i >>= 1; i >= 1; i >>= 1) node_[i] = std::min(node_[2 * i], node_[2 * i + 1]);
+}
+
 bool MidiFile::next(kst_midi_event &ev) {  // src/midi.rs:351-361
-    if (heap_.empty()) return false;
-    const auto [tick, idx] = heap_.top();
-    heap_.pop();
-    ev = pending_[idx];
-    ev.tick = tick;
-    Track &t = tracks_[idx];
-    if (t.next_event(pending_[idx])) heap_.emplace(t.tick, idx);
+    if (run_) {
+        Track &t = tracks_[run_w_];
+        if (t.at < t.n || t.refill()) {
+            const uint64_t tick = t.ev[t.at].tick;
+            if (tick < (1ull << 44) && ((tick << 20) | run_w_) < run_limit_) {
+                ev = t.ev[t.at++];
+                return true;
+            }
+        }
+        run_ = false;
+        update(run_w_);
+    }
+    if (!wide_ && node_[1] == UINT64_MAX - 1) wide_ = true;
+    if (wide_) {
+        const uint32_t w = scan_min();
+        if (w == UINT32_MAX) return false;
+        Track &t = tracks_[w];
+        ev = t.ev[t.at++];
+        if (t.at == t.n) t.refill();
+        return true;
+    }
+    if (node_[1] == UINT64_MAX) return false;
+    const uint32_t w = (uint32_t)(node_[1] & ((1u << 20) - 1));
+    uint64_t limit = UINT64_MAX;
+    for (uint32_t i = leaves_ + w; i > 1; i >>= 1) limit = std::min(limit, node_[i ^ 1]);
+    Track &t = tracks_[w];
+    ev = t.ev[t.at++];
+    run_ = true;
+    run_w_ = w;
+    run_limit_ = limit;
     return true;
 }
 
@@ -264,6 +326,56 @@ bool MidiFile::Track::next_event(kst_midi_event &ev) {  // src/midi.rs:128-247
         return true;
     }
     return true;  // other system messages: one data byte already taken
+}
+
+Prefetch::Prefetch(std::unique_ptr<EventSource> inner, size_t chunk, size_t depth)
+    : inner_(std::move(inner)), chunk_(chunk), depth_(depth) {
+    division = inner_->division;
+    th_ = std::thread([this] { run(); });
+}
+
+Prefetch::~Prefetch() {
+    {
+        std::lock_guard<std::mutex> g(m_);
+        stop_ = true;
+    }
+    cv_.notify_all();
+    th_.join();
+}
+
+void Prefetch::run() {
+    for (;;) {
+        std::vector<kst_midi_event> c;
+        c.reserve(chunk_);
+        kst_midi_event e;
+        while (c.size() < chunk_ && inner_->next(e)) c.push_back(e);
+        const bool last = c.size() < chunk_;
+        std::unique_lock<std::mutex> g(m_);
+        cv_.wait(g, [&] { return stop_ || q_.size() < depth_; });
+        if (stop_) return;
+        if (!c.empty()) q_.push_back(std::move(c));
+        if (last) done_ = true;
+        g.unlock();
+        cv_.notify_all();
+        if (last) return;
+    }
+}
+
+bool Prefetch::next(kst_midi_event &ev) {
+    if (at_ < cur_.size()) {
+        ev = cur_[at_++];
+        return true;
+    }
+    std::unique_lock<std::mutex> g(m_);
+    cv_.wait(g, [&] { return !q_.empty() || done_; });
+    if (q_.empty()) return false;
+    cur_ = std::move(q_.front());
+    q_.pop_front();
+    g.unlock();
+    cv_.notify_all();
+    at_ = 0;
+    ev = cur_[at_++];
+    return true;
 }
 
 TempoClock::TempoClock(const kst_division &d, uint32_t sample_rate)

@@ -240,6 +240,12 @@ class Driver:
         _check(self._lib.kst_driver_get_stats(self._h, C.byref(s)))
         return s.as_dict()
 
+    def timings(self) -> dict:
+        """Host milliseconds per block of next_block, by phase."""
+        out = (abi.KstTiming * 8)()
+        n = self._lib.kst_driver_timings(self._h, out, 8)
+        return {out[i].name.decode(): out[i].ms for i in range(min(n, 8))}
+
     def seconds_rendered(self) -> float:
         return self.stats()["frames"] / self.cfg.sample_rate
 

@@ -827,3 +827,111 @@ def synthetic_pool(cfg: Config, bank: Bank, n: int, seed: int, chunk: int = 1 <<
         f[FIELD["note_lo"]] = (ids & np.uint64(0xFFFFFFFF)).astype(np.uint32)
         f[FIELD["note_hi"]] = (ids >> np.uint64(32)).astype(np.uint32)
     return out
+
+
+# ---------------------------------------------------------------------------
+# inputs of the native driver (kestrel_amd/driver.py): a bank description in
+# the C ABI's region layout and a dense standard MIDI file
+# ---------------------------------------------------------------------------
+
+def driver_bank_desc(seed: int, instruments: int = 16, zones: int = 4, frames: int = 32768,
+                     adsr=(0.005, 0.100, 0.6, 0.150), stereo: bool = False):
+    """(pool, samples, regions, presets, preset_regions): `make_bank`'s samples as
+    SampleInfo / Region records (one preset per instrument, program = instrument;
+    the last one also as the drum kit in bank 128), filters on half the regions.
+    Stereo banks get two hard-panned regions per zone, as src/sf2.rs lays out a
+    stereo pair."""
+    from .abi import PRESET_DTYPE, REGION_DTYPE, SAMPLE_DTYPE
+    b = make_bank(seed, instruments=instruments, zones=zones, key_lo=0, key_hi=127,
+                  frames=frames, adsr=adsr, stereo=stereo)
+    rng = np.random.default_rng(seed + 7)
+    n = len(b.smp_start)
+    chans = 2 if stereo else 1
+    samples = np.zeros(n * chans, SAMPLE_DTYPE)
+    regions = np.zeros(n * chans, REGION_DTYPE)
+    a, d, s, r = adsr
+    for i in range(n):
+        for c in range(chans):
+            j = i * chans + c
+            samples[j] = ((b.smp_start_r if c else b.smp_start)[i], b.smp_len[i], b.loop_start[i],
+                          b.loop_end[i], 48000, 0.0, 1.0, int(b.root_key[i]))
+            reg = regions[j:j + 1]
+            reg["sample"] = j
+            reg["key_lo"], reg["key_hi"] = int(b.key_lo[i]), int(b.key_hi[i])
+            reg["vel_lo"], reg["vel_hi"] = 0, 127
+            reg["root_key_override"] = reg["fixed_key"] = reg["fixed_vel"] = -1
+            reg["scale_tuning"] = 100
+            reg["amp_veltrack"] = 100.0
+            reg["pan"] = (-1.0 if c == 0 else 1.0) if stereo else float(rng.uniform(-0.5, 0.5))
+            reg["loop_mode"] = 2 if b.loop_flags[i] else 0
+            reg["attack"], reg["decay"], reg["sustain"], reg["release"] = a, d, s, r
+            reg["attenuation_cb"] = float(rng.uniform(0, 60))
+            reg["filter_fc_cents"] = float(b.design["fc_cents"][i]) if i % 2 else 13500.0
+            reg["filter_q_cb"] = float(b.design["q_cb"][i])
+    presets = np.zeros(instruments + 1, PRESET_DTYPE)
+    per = zones * chans
+    for p in range(instruments):
+        presets[p] = (0, p, p * per, per)
+    presets[instruments] = (128, 0, (instruments - 1) * per, per)
+    return (b.pool, samples, regions, presets, np.arange(n * chans, dtype=np.uint32))
+
+
+def dense_midi(path: str, seed: int, notes: int, seconds: float, ppq: int = 960,
+               channels: int = 16, note_ms=(2.0, 60.0), key_range=(12, 115)) -> int:
+    """Write a format-1 SMF of `notes` short notes spread uniformly over
+    `seconds` at 120 bpm, one track per channel, each opening with a program
+    change, volume, pan and expression; returns the file size. Events are
+    written with running status. Vectorised, so tens of millions of notes
+    take seconds."""
+    rng = np.random.default_rng(seed)
+    tps = ppq * 2.0  # ticks per second at 120 bpm
+    on = np.sort(rng.uniform(0, seconds * tps, notes)).astype(np.int64)
+    dur = np.maximum(1, (rng.uniform(*note_ms, notes) * 1e-3 * tps).astype(np.int64))
+    ch = rng.integers(0, channels, notes)
+    key = rng.integers(key_range[0], key_range[1], notes)
+    vel = rng.integers(30, 128, notes)
+    tracks = []
+    for c in range(channels):
+        m = ch == c
+        t_on, t_off, k, v = on[m], on[m] + dur[m], key[m], vel[m]
+        tick = np.concatenate([t_on, t_off])
+        kind = np.concatenate([np.ones(t_on.size, np.int64), np.zeros(t_off.size, np.int64)])
+        kk = np.concatenate([k, k])
+        vv = np.concatenate([v, np.zeros_like(v)])  # note-off as note-on velocity 0
+        order = np.lexsort((kind, tick))  # offs before ons on a shared tick
+        tick, kk, vv = tick[order], kk[order], vv[order]
+        delta = np.diff(np.concatenate([[0], tick]))
+        head = bytes([0, 0xC0 | c, c % 8, 0, 0xB0 | c, 7, 110, 0, 10, int(rng.integers(20, 108)),
+                      0, 11, 120])
+        body = _varlen_bytes(delta, kk, vv, 0x90 | c)
+        tracks.append(head + body + bytes([0, 0xFF, 0x2F, 0]))
+    with open(path, "wb") as f:
+        f.write(b"MThd" + (6).to_bytes(4, "big") + (1).to_bytes(2, "big")
+                + len(tracks).to_bytes(2, "big") + ppq.to_bytes(2, "big"))
+        for t in tracks:
+            f.write(b"MTrk" + len(t).to_bytes(4, "big") + t)
+        return f.tell()
+
+
+def _varlen_bytes(delta: np.ndarray, key: np.ndarray, vel: np.ndarray, status: int) -> bytes:
+    """Delta-time varlen + running-status note messages, vectorised: the first
+    message carries the status byte, the rest reuse it."""
+    delta = delta.astype(np.int64)
+    nb = np.ones(delta.size, np.int64)
+    for k in range(1, 5):
+        nb += delta >= (1 << (7 * k))
+    total = nb + 2
+    total[0] += 1
+    out = np.zeros(int(total.sum()), np.uint8)
+    pos = np.concatenate([[0], np.cumsum(total)[:-1]])
+    for k in range(5):
+        sel = nb > k
+        shift = 7 * (nb[sel] - 1 - k)
+        byte = (delta[sel] >> shift) & 0x7F
+        out[pos[sel] + k] = byte | np.where(k < nb[sel] - 1, 0x80, 0)
+    p = pos + nb
+    out[p[0]] = status
+    p[0] += 1
+    out[p] = key
+    out[p + 1] = vel
+    return out.tobytes()
