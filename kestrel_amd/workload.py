@@ -1,11 +1,11 @@
 """Synthetic banks, note streams and a driver-lite that turns them into the
 per-block inputs of the `Backend` trait: gate rows, channel rows and spawn
-commands.
+commands, for the backend-only workloads; and the inputs of the native driver
+(kestrel_amd/driver.py): a bank description and dense SMF files.
 
-The reference's `Driver::next_block` (src/driver.rs:434-642) is the caller of
-the hot path and stays out of scope; this module reproduces the parts of it the
-backend's inputs depend on, so the CPU oracle and the gfx950 backend are fed
-identical blocks:
+The driver-lite reproduces the parts of `Driver::next_block` (src/driver.rs:
+434-642) the backend's inputs depend on, vectorised for millions of notes, so
+the CPU oracle and the gfx950 backend are fed identical blocks:
 
 * GateTable / ChannelTable row semantics (src/voice.rs:219-658), ported
   faithfully (`GateTable`, `ChannelTable`) and, for large note counts, built
