@@ -330,6 +330,28 @@ int kst_bank_variant(const kst_bank *b, const uint8_t cc[5], kst_region_params *
 int64_t kst_bank_find_preset(const kst_bank *b, uint32_t bank, uint32_t program);
 void kst_bank_destroy(kst_bank *b);
 
+/* The bank's flattened contents, as created or loaded (valid while it lives). */
+int kst_bank_contents(const kst_bank *b, const kst_sample_info **samples, size_t *n_samples,
+                      const kst_region **regions, size_t *n_regions, const kst_preset **presets,
+                      size_t *n_presets, const uint32_t **preset_regions,
+                      size_t *n_preset_regions, uint32_t *pool_rate);
+
+/* Pool building of the loaders, src/config.rs:319-322 (defaults :397-398). */
+typedef struct kst_load_config {
+    uint32_t resample_pool;      /* convert every used sample to the output rate */
+    uint32_t pad;
+    uint64_t sample_pool_budget; /* bytes; the pool rate halves until it fits */
+} kst_load_config;
+void kst_default_load_config(kst_load_config *cfg);
+/* sf2::load, src/sf2.rs:221-506: every preset flattened into resolved regions,
+ * only the samples some region uses read into the pool (src/sf2.rs:585-703),
+ * then the bank built as kst_bank_create builds it. */
+int kst_bank_load_sf2(const kst_config *cfg, const kst_driver_config *dcfg,
+                      const kst_load_config *lcfg, const char *path, kst_bank **out);
+/* resample_i16, src/resample.rs:102-108: writes up to `cap` samples and
+ * returns the converted length (call with cap 0 to size `out`). */
+int64_t kst_resample_i16(const int16_t *src, size_t n, double ratio, int16_t *out, size_t cap);
+
 /* Division, src/midi.rs:32-36: smpte == 0 uses ppq, else fps * ticks_per_frame. */
 typedef struct kst_division {
     uint32_t smpte, ppq, fps, ticks_per_frame;

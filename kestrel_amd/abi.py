@@ -142,6 +142,10 @@ SIGNATURES = {
     "kst_bank_variant": (i32, [P, P, P, sz]),
     "kst_bank_find_preset": (i64, [P, u32, u32]),
     "kst_bank_destroy": (None, [P]),
+    "kst_bank_contents": (i32, [P, P, P, P, P, P, P, P, P, P]),
+    "kst_default_load_config": (None, [P]),
+    "kst_bank_load_sf2": (i32, [P, P, P, C.c_char_p, P]),
+    "kst_resample_i16": (i64, [P, sz, C.c_double, P, sz]),
     "kst_midi_open": (i32, [C.c_char_p, P]),
     "kst_midi_info": (i32, [P, P, P, P]),
     "kst_midi_read": (i64, [P, P, sz]),

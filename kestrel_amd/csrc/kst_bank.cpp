@@ -98,7 +98,8 @@ Bank::Bank(const BankSettings &s, const int16_t *pool_in, size_t pool_len, uint3
            const kst_preset *pre, size_t n_presets, const uint32_t *pre_regions,
            size_t n_pre_regions)
     : pool(pool_in, pool_in + pool_len), pool_rate(pool_rate_in), samples(smp, smp + n_samples),
-      regions(reg, reg + n_regions), set(s) {
+      regions(reg, reg + n_regions), set(s), preset_desc(pre, pre + n_presets),
+      preset_regions(pre_regions, pre_regions + n_pre_regions) {
     if (set.sample_rate == 0) throw std::invalid_argument("sample_rate must be non-zero");
     for (size_t i = 0; i < n_samples; ++i) {
         const kst_sample_info &x = samples[i];

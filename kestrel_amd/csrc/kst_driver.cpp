@@ -1064,6 +1064,54 @@ int64_t kst_bank_find_preset(const kst_bank *b, uint32_t bank, uint32_t program)
 
 void kst_bank_destroy(kst_bank *b) { delete b; }
 
+int kst_bank_contents(const kst_bank *b, const kst_sample_info **samples, size_t *n_samples,
+                      const kst_region **regions, size_t *n_regions, const kst_preset **presets,
+                      size_t *n_presets, const uint32_t **preset_regions,
+                      size_t *n_preset_regions, uint32_t *pool_rate) {
+    if (!b) return fail(KST_EINVAL, "null bank");
+    const kst::Bank &k = b->bank;
+    if (samples) *samples = k.samples.data();
+    if (n_samples) *n_samples = k.samples.size();
+    if (regions) *regions = k.regions.data();
+    if (n_regions) *n_regions = k.regions.size();
+    if (presets) *presets = k.preset_desc.data();
+    if (n_presets) *n_presets = k.preset_desc.size();
+    if (preset_regions) *preset_regions = k.preset_regions.data();
+    if (n_preset_regions) *n_preset_regions = k.preset_regions.size();
+    if (pool_rate) *pool_rate = k.pool_rate;
+    return KST_OK;
+}
+
+void kst_default_load_config(kst_load_config *c) {
+    if (!c) return;
+    c->resample_pool = 1;
+    c->pad = 0;
+    c->sample_pool_budget = 2ull << 30;
+}
+
+int kst_bank_load_sf2(const kst_config *cfg, const kst_driver_config *dcfg,
+                      const kst_load_config *lcfg, const char *path, kst_bank **out) {
+    if (!cfg || !dcfg || !lcfg || !path || !out) return fail(KST_EINVAL, "null argument");
+    if (cfg->sample_rate == 0) return fail(KST_EINVAL, "sample_rate must be non-zero");
+    try {
+        std::unique_ptr<kst::Bank> bank(kst::load_sf2(bank_settings(*cfg, *dcfg), path,
+                                                      lcfg->resample_pool != 0,
+                                                      lcfg->sample_pool_budget));
+        *out = new kst_bank{std::move(*bank)};
+        return KST_OK;
+    } catch (const std::exception &e) {
+        return fail(KST_EINVAL, e.what());
+    }
+}
+
+int64_t kst_resample_i16(const int16_t *src, size_t n, double ratio, int16_t *out, size_t cap) {
+    if ((!src && n) || (!out && cap)) return fail(KST_EINVAL, "null argument");
+    if (!(ratio > 0.0) || !std::isfinite(ratio)) return fail(KST_EINVAL, "ratio must be positive");
+    const std::vector<int16_t> v = kst::resample_i16(src, n, ratio);
+    std::copy(v.begin(), v.begin() + (ptrdiff_t)std::min(cap, v.size()), out);
+    return (int64_t)v.size();
+}
+
 int kst_midi_open(const char *path, kst_midi **out) {
     if (!path || !out) return fail(KST_EINVAL, "null argument");
     try {

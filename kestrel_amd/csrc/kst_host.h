@@ -82,6 +82,8 @@ public:
     std::vector<uint32_t> params_base, params_stride, params_vel_span;
     std::vector<kst_region_params> params;  // variant 0
     uint32_t max_sample_len = 0;
+    std::vector<kst_preset> preset_desc;      // as given: regions are
+    std::vector<uint32_t> preset_regions;     // preset_regions[first .. first+count]
 
     // Builds the layout, variant 0 and the admission tables. Throws
     // std::invalid_argument naming what is wrong with the input.
@@ -198,6 +200,12 @@ private:
     std::vector<kst_midi_event> events_;
     size_t at_ = 0;
 };
+
+// src/sf2.rs: load a SoundFont 2 file into a bank (throws std::runtime_error
+// with the reference's message). src/resample.rs: the pool's rate converter.
+Bank *load_sf2(const BankSettings &set, const std::string &path, bool resample_pool,
+               uint64_t budget);
+std::vector<int16_t> resample_i16(const int16_t *src, size_t n, double ratio);
 
 // Decodes another source ahead on a thread of its own, in chunks through a
 // bounded queue: the consumer sees exactly the inner sequence, and the file's

@@ -50,6 +50,22 @@ class DriverConfig(C.Structure):
         return d
 
 
+class LoadConfig(C.Structure):
+    """`kst_load_config`: resample_pool and sample_pool_budget, src/config.rs:319-322."""
+    _fields_ = [("resample_pool", C.c_uint32), ("pad", C.c_uint32),
+                ("sample_pool_budget", C.c_uint64)]
+
+
+def resample_i16(src: np.ndarray, ratio: float) -> np.ndarray:
+    """resample_i16, src/resample.rs:102-108."""
+    lib = abi.load()
+    s = _arr(src, np.int16)
+    n = _check(lib.kst_resample_i16(abi.ptr(s), s.size, float(ratio), None, 0))
+    out = np.zeros(n, np.int16)
+    _check(lib.kst_resample_i16(abi.ptr(s), s.size, float(ratio), abi.ptr(out), out.size))
+    return out
+
+
 class Division(C.Structure):
     _fields_ = [("smpte", C.c_uint32), ("ppq", C.c_uint32), ("fps", C.c_uint32),
                 ("ticks_per_frame", C.c_uint32)]
@@ -85,29 +101,55 @@ class Bank:
     and the admission preview tables, with `cfg`'s rate, curves and volume."""
 
     def __init__(self, cfg: Config, pool: np.ndarray, samples: np.ndarray, regions: np.ndarray,
-                 presets: np.ndarray, preset_regions: np.ndarray, pool_rate: int = 0):
+                 presets: np.ndarray, preset_regions: np.ndarray, pool_rate: int = 0, *,
+                 _handle=None):
         self._lib = abi.load()
         self.cfg = cfg
-        self._pool = _arr(pool, np.int16)
-        self._samples = _arr(samples, SAMPLE_DTYPE)
-        self._regions = _arr(regions, REGION_DTYPE)
-        self._presets = _arr(presets, PRESET_DTYPE)
-        self._pre_regions = _arr(preset_regions, np.uint32)
-        k, d = KstConfig.from_config(cfg), DriverConfig.from_config(cfg)
         h = C.c_void_p()
-        _check(self._lib.kst_bank_create(
-            C.byref(k), C.byref(d), abi.ptr(self._pool), self._pool.size, int(pool_rate),
-            abi.ptr(self._samples), self._samples.size, abi.ptr(self._regions),
-            self._regions.size, abi.ptr(self._presets), self._presets.size,
-            abi.ptr(self._pre_regions), self._pre_regions.size, C.byref(h)))
+        if _handle is not None:
+            h = _handle
+        else:
+            pool_a = _arr(pool, np.int16)
+            smp, reg = _arr(samples, SAMPLE_DTYPE), _arr(regions, REGION_DTYPE)
+            pre, pre_r = _arr(presets, PRESET_DTYPE), _arr(preset_regions, np.uint32)
+            k, d = KstConfig.from_config(cfg), DriverConfig.from_config(cfg)
+            _check(self._lib.kst_bank_create(
+                C.byref(k), C.byref(d), abi.ptr(pool_a), pool_a.size, int(pool_rate),
+                abi.ptr(smp), smp.size, abi.ptr(reg), reg.size, abi.ptr(pre), pre.size,
+                abi.ptr(pre_r), pre_r.size, C.byref(h)))
         self._h = h
         pl, npar, nreg, msl = C.c_size_t(), C.c_size_t(), C.c_uint32(), C.c_uint32()
         _check(self._lib.kst_bank_info(h, C.byref(pl), C.byref(npar), C.byref(nreg), C.byref(msl)))
         self.n_params, self.n_regions, self.max_sample_len = npar.value, nreg.value, msl.value
-        self.pool = self._pool
-        self.params = np.ctypeslib.as_array(
-            C.cast(self._lib.kst_bank_params(h), C.POINTER(C.c_uint8)),
-            shape=(self.n_params * PARAMS_DTYPE.itemsize,)).view(PARAMS_DTYPE).copy()
+        self.pool = _copy(C.c_void_p(self._lib.kst_bank_pool(h)), pl.value, np.int16)
+        self.params = _copy(C.c_void_p(self._lib.kst_bank_params(h)), self.n_params, PARAMS_DTYPE)
+
+    @classmethod
+    def load_sf2(cls, cfg: Config, path: str, resample_pool: bool = True,
+                 sample_pool_budget: int = 2 << 30) -> "Bank":
+        """sf2::load (src/sf2.rs:221-506) with the pool settings of src/config.rs:319-322."""
+        lib = abi.load()
+        k, d = KstConfig.from_config(cfg), DriverConfig.from_config(cfg)
+        lc = LoadConfig(int(resample_pool), 0, int(sample_pool_budget))
+        h = C.c_void_p()
+        _check(lib.kst_bank_load_sf2(C.byref(k), C.byref(d), C.byref(lc), str(path).encode(),
+                                     C.byref(h)))
+        return cls(cfg, None, None, None, None, None, _handle=h)
+
+    def contents(self) -> dict:
+        """The flattened samples, regions, presets and their region lists, and the pool rate."""
+        ptrs = [C.c_void_p() for _ in range(4)]
+        ns = [C.c_size_t() for _ in range(4)]
+        rate = C.c_uint32()
+        _check(self._lib.kst_bank_contents(self._h, C.byref(ptrs[0]), C.byref(ns[0]),
+                                           C.byref(ptrs[1]), C.byref(ns[1]), C.byref(ptrs[2]),
+                                           C.byref(ns[2]), C.byref(ptrs[3]), C.byref(ns[3]),
+                                           C.byref(rate)))
+        return {"samples": _copy(ptrs[0], ns[0].value, SAMPLE_DTYPE),
+                "regions": _copy(ptrs[1], ns[1].value, REGION_DTYPE),
+                "presets": _copy(ptrs[2], ns[2].value, PRESET_DTYPE),
+                "preset_regions": _copy(ptrs[3], ns[3].value, np.uint32),
+                "pool": self.pool, "pool_rate": rate.value}
 
     def variant(self, cc71: int, cc72: int, cc73: int, cc74: int, cc75: int) -> np.ndarray:
         """Bank::build_variant for ParamMod::from_controllers (src/bank.rs:336-350, 836-853)."""

@@ -86,3 +86,59 @@ def cpu_params_dtype():
 def spawn_dtype():
     from kestrel_amd.abi import SPAWN_DTYPE
     return SPAWN_DTYPE
+
+
+def test_soundfont_through_driver_and_device(tmp_path):
+    """An SF2 file loaded by the library (pool resampled to the output rate)
+    and a MIDI file through its driver and the device, against the same
+    bank contents through the oracle's driver and CPU backend."""
+    import sf2_cases as S
+    from kestrel_amd.driver import Bank, Driver
+    from kestrel_amd.backend import HipSynth
+    from oracle import driver as OD
+    from oracle import oracle
+
+    cfg = DC.driver_config(block_frames=256, gate_frames=32, max_voices=300, max_layers=4,
+                           compensated=True)
+    sf2 = str(tmp_path / "bank.sf2")
+    S.random_sf2(sf2, 3)
+    lb = Bank.load_sf2(cfg, sf2)
+    c = lb.contents()
+    ob = OD.Bank(DC.oracle_settings(cfg), c["samples"], c["regions"], c["presets"],
+                 c["preset_regions"], pool_rate=c["pool_rate"])
+    ev = DC.random_events(5, 2500, 500, cc_rate=0.2, channels=range(8))
+    mid = str(tmp_path / "song.mid")
+    DC.write_smf(mid, [DC.track_bytes(ev)], ppq=480)
+    gpu = HipSynth(cfg, lb.pool, lb.params, lb.n_regions, lb.max_sample_len)
+    cpu = oracle.Oracle(cfg, lb.pool, lb.params)
+    cpu.set_sorted(lb.n_regions, lb.max_sample_len, cfg.sort_voices)
+    ld = Driver.open(cfg, lb, mid)
+    div, oev = OD.read_midi(mid)
+    od = OD.Driver(cfg, ob, div, oev)
+    out = np.zeros(cfg.block_frames * 2, np.float32)
+    worst, live = -math.inf, 0
+    for b in range(220):
+        print(f"sf2 block {b}", flush=True)
+        more = ld.next_block(gpu, out)
+        for idx, table in od.begin_block():
+            cpu.set_params_variant(idx, OD.params_array(table, cpu_params_dtype()))
+        cpu.set_gates(od.gates.rows.reshape(-1))
+        cpu.set_channels(od.chan.rows.reshape(-1), *od.flags)
+        st = cpu.stats()
+        cpu.spawn(OD.spawn_array(od.admit(st["active_voices"], st["stolen"]), spawn_dtype()))
+        stolen_after = cpu.stats()["stolen"]
+        oc = cpu.render()
+        assert more == od.end_block(oc, stolen_after, cpu.stats()), b
+        pg, pc = gpu.pool_export(), cpu.pool_export()
+        assert pg.shape == pc.shape and np.array_equal(pg, pc), f"block {b}: pool"
+        peak = float(np.abs(oc).max())
+        if peak > 0:
+            d = float(np.abs(out.astype(np.float64) - oc).max())
+            worst = max(worst, -math.inf if d == 0 else 20 * math.log10(d / peak))
+        live = max(live, gpu.stats()["active_voices"])
+        if not more:
+            break
+    print(f"sf2: {lb.n_regions} regions, pool {c['pool'].size} frames at {c['pool_rate']} Hz, "
+          f"peak live {live}, worst null {worst:.1f} dB")
+    assert live > 100 and worst <= NULL_DB
+    gpu.close()
