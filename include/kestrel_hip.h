@@ -348,6 +348,13 @@ void kst_default_load_config(kst_load_config *cfg);
  * then the bank built as kst_bank_create builds it. */
 int kst_bank_load_sf2(const kst_config *cfg, const kst_driver_config *dcfg,
                       const kst_load_config *lcfg, const char *path, kst_bank **out);
+/* sfz::load, src/sfz.rs:254-425: <control>/<global>/<master>/<group>/<region>
+ * sections with #include, every referenced WAV (src/wav.rs:221-365) read once
+ * per channel, stereo files as two hard-panned regions, all in one preset
+ * (bank 0, program 0). The pool is converted to the output rate when
+ * resample_pool is set; the budget does not apply to SFZ. */
+int kst_bank_load_sfz(const kst_config *cfg, const kst_driver_config *dcfg,
+                      const kst_load_config *lcfg, const char *path, kst_bank **out);
 /* resample_i16, src/resample.rs:102-108: writes up to `cap` samples and
  * returns the converted length (call with cap 0 to size `out`). */
 int64_t kst_resample_i16(const int16_t *src, size_t n, double ratio, int16_t *out, size_t cap);

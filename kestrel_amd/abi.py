@@ -145,6 +145,7 @@ SIGNATURES = {
     "kst_bank_contents": (i32, [P, P, P, P, P, P, P, P, P, P]),
     "kst_default_load_config": (None, [P]),
     "kst_bank_load_sf2": (i32, [P, P, P, C.c_char_p, P]),
+    "kst_bank_load_sfz": (i32, [P, P, P, C.c_char_p, P]),
     "kst_resample_i16": (i64, [P, sz, C.c_double, P, sz]),
     "kst_midi_open": (i32, [C.c_char_p, P]),
     "kst_midi_info": (i32, [P, P, P, P]),

@@ -22,7 +22,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB_NAME = "libkestrel_hip.so"
 LIB_PATH = os.path.join(HERE, LIB_NAME)
 SOURCES = ["kst_backend.hip", "kst_pool.hip", "kst_render.hip", "kst_design.hip", "kst_post.cpp",
-           "kst_bank.cpp", "kst_midi.cpp", "kst_driver.cpp", "kst_sf2.cpp"]
+           "kst_bank.cpp", "kst_midi.cpp", "kst_driver.cpp", "kst_sf2.cpp",
+           "kst_sfz.cpp"]
 ARCH = os.environ.get("KST_OFFLOAD_ARCH", "gfx950")
 
 

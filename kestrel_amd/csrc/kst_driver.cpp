@@ -1104,6 +1104,20 @@ int kst_bank_load_sf2(const kst_config *cfg, const kst_driver_config *dcfg,
     }
 }
 
+int kst_bank_load_sfz(const kst_config *cfg, const kst_driver_config *dcfg,
+                      const kst_load_config *lcfg, const char *path, kst_bank **out) {
+    if (!cfg || !dcfg || !lcfg || !path || !out) return fail(KST_EINVAL, "null argument");
+    if (cfg->sample_rate == 0) return fail(KST_EINVAL, "sample_rate must be non-zero");
+    try {
+        std::unique_ptr<kst::Bank> bank(
+            kst::load_sfz(bank_settings(*cfg, *dcfg), path, lcfg->resample_pool != 0));
+        *out = new kst_bank{std::move(*bank)};
+        return KST_OK;
+    } catch (const std::exception &e) {
+        return fail(KST_EINVAL, e.what());
+    }
+}
+
 int64_t kst_resample_i16(const int16_t *src, size_t n, double ratio, int16_t *out, size_t cap) {
     if ((!src && n) || (!out && cap)) return fail(KST_EINVAL, "null argument");
     if (!(ratio > 0.0) || !std::isfinite(ratio)) return fail(KST_EINVAL, "ratio must be positive");

@@ -206,6 +206,8 @@ private:
 Bank *load_sf2(const BankSettings &set, const std::string &path, bool resample_pool,
                uint64_t budget);
 std::vector<int16_t> resample_i16(const int16_t *src, size_t n, double ratio);
+// src/sfz.rs: load an SFZ instrument (and the WAV files it names) into a bank.
+Bank *load_sfz(const BankSettings &set, const std::string &path, bool resample_pool);
 
 // Decodes another source ahead on a thread of its own, in chunks through a
 // bounded queue: the consumer sees exactly the inner sequence, and the file's

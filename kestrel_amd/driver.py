@@ -136,6 +136,17 @@ class Bank:
                                      C.byref(h)))
         return cls(cfg, None, None, None, None, None, _handle=h)
 
+    @classmethod
+    def load_sfz(cls, cfg: Config, path: str, resample_pool: bool = True) -> "Bank":
+        """sfz::load (src/sfz.rs:254-425)."""
+        lib = abi.load()
+        k, d = KstConfig.from_config(cfg), DriverConfig.from_config(cfg)
+        lc = LoadConfig(int(resample_pool), 0, 0)
+        h = C.c_void_p()
+        _check(lib.kst_bank_load_sfz(C.byref(k), C.byref(d), C.byref(lc), str(path).encode(),
+                                     C.byref(h)))
+        return cls(cfg, None, None, None, None, None, _handle=h)
+
     def contents(self) -> dict:
         """The flattened samples, regions, presets and their region lists, and the pool rate."""
         ptrs = [C.c_void_p() for _ in range(4)]
