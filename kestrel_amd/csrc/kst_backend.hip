@@ -195,7 +195,7 @@ struct kst_backend {
     std::vector<uint32_t> h_gates, h_chans;
     uint32_t h_chan_flags = 0;
     bool staged_chan = true, graph_chan = true;
-    std::vector<kst_spawn_cmd> pending;
+    size_t n_pending = 0;  // this block's spawn commands, staged in pin_cmds
     uint32_t *pin_small = nullptr;  // pinned staging for per-block uploads
     size_t pin_cmds_cap = 0;
     kst_spawn_cmd *pin_cmds = nullptr;
@@ -680,7 +680,17 @@ int kst_spawn(kst_backend *b, const kst_spawn_cmd *cmds, size_t n) {
     if (b && b->inflight) return fail(KST_ESTATE, "a submitted render has not been collected");
     if (!b || (!cmds && n)) return fail(KST_EINVAL, "null argument");
     if (n > 0xFFFFFFFFull) return fail(KST_EINVAL, "%zu spawn commands in one block", n);
-    b->pending.assign(cmds, cmds + n);
+    // Straight into the pinned staging buffer the render uploads from.
+    if (n > b->pin_cmds_cap) {
+        if (b->pin_cmds) (void)hipHostFree(b->pin_cmds);
+        b->pin_cmds = nullptr;
+        b->pin_cmds_cap = 0;
+        const size_t cap = std::max<size_t>(n + n / 4, 1024);
+        HIPCHK(hipHostMalloc(&b->pin_cmds, cap * sizeof(kst_spawn_cmd)));
+        b->pin_cmds_cap = cap;
+    }
+    if (n) std::memcpy(b->pin_cmds, cmds, n * sizeof(kst_spawn_cmd));
+    b->n_pending = n;
     // Eager accounting, src/cpu.rs:330-390; the device repeats this arithmetic
     // in k_begin from the same inputs.
     const uint64_t cap = b->cfg.max_voices, live = b->live, raw = n;
@@ -699,27 +709,18 @@ int kst_render_submit(kst_backend *b) {
     if (b->inflight) return fail(KST_ESTATE, "a submitted render has not been collected");
     const size_t two_f = 2 * (size_t)b->cfg.block_frames;
     HIPCHK(hipSetDevice(b->device));
-    const uint32_t ncmd = (uint32_t)b->pending.size();
+    const uint32_t ncmd = (uint32_t)b->n_pending;
     int rc = b->ensure_stage(1, ncmd);
     if (rc) return rc;
-    if (ncmd > b->pin_cmds_cap) {
-        if (b->pin_cmds) (void)hipHostFree(b->pin_cmds);
-        b->pin_cmds = nullptr;
-        size_t cap = std::max<size_t>(ncmd, 1024);
-        HIPCHK(hipHostMalloc(&b->pin_cmds, cap * sizeof(kst_spawn_cmd)));
-        b->pin_cmds_cap = cap;
-    }
     hipStream_t s = b->stream;
     // Block slot 0: gates, channels, flags, commands, and the block index.
     HIPCHK(hipMemcpyAsync(b->g_gates.p, b->h_gates.data(), b->gate_words * 4,
                           hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(b->g_chans.p, b->h_chans.data(), b->chan_words * 4,
                           hipMemcpyHostToDevice, s));
-    if (ncmd) {
-        std::memcpy(b->pin_cmds, b->pending.data(), (size_t)ncmd * sizeof(kst_spawn_cmd));
+    if (ncmd)
         HIPCHK(hipMemcpyAsync(b->g_cmds.p, b->pin_cmds, (size_t)ncmd * sizeof(kst_spawn_cmd),
                               hipMemcpyHostToDevice, s));
-    }
     b->pin_small[0] = b->h_chan_flags;
     b->pin_small[1] = ncmd;
     b->pin_small[2] = 0;  // command offset
@@ -768,7 +769,7 @@ int kst_render_collect(kst_backend *b, float *out, size_t n) {
                     (unsigned long long)r.stolen, (unsigned long long)r.dropped,
                     (unsigned long long)b->stolen, (unsigned long long)b->dropped);
     if (b->profile) b->read_timings(b->ev, 1);
-    b->pending.clear();
+    b->n_pending = 0;
     return KST_OK;
 }
 
@@ -785,9 +786,9 @@ int kst_stats(const kst_backend *b, kst_block_stats *out) {
     // After spawn and before render the CPU reference counts the appended
     // voices (src/cpu.rs:641-648); so does this.
     uint64_t active = b->live;
-    if (!b->pending.empty()) {
+    if (b->n_pending) {
         const uint64_t cap = b->cfg.max_voices, live = b->live;
-        const uint64_t want = std::min<uint64_t>(b->pending.size(), cap);
+        const uint64_t want = std::min<uint64_t>(b->n_pending, cap);
         uint64_t need = 0;
         if (live + want > cap && b->cfg.steal_rule != STEAL_DROP_NEW)
             need = std::min<uint64_t>(std::min<uint64_t>(live + want - cap, live), b->max_steal);
