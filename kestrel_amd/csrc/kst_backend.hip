@@ -204,6 +204,9 @@ struct kst_backend {
 
     // Host mirror, exact: every change is host-decided or read back.
     uint32_t live = 0, parity = 0;
+    // A fused block left the live pool behind its sort's permutation (the
+    // device's DevState::perm_valid, mirrored here).
+    bool perm_valid = false;
     uint64_t stolen = 0, dropped = 0;
     float peak = 0.0f;
 
@@ -277,6 +280,20 @@ struct kst_backend {
         return KST_OK;
     }
 
+    // Apply a pending permutation as a copy, so the live pool is in order in
+    // one buffer: before the host reads or writes it, and before a batch (which
+    // compacts with its own gather) runs this render.
+    int settle() {
+        if (!perm_valid) return KST_OK;
+        Args a = args(1);
+        launch_settle(a, per_voice_grid(slots), stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(stream));
+        parity ^= 1;
+        perm_valid = false;
+        return KST_OK;
+    }
+
     void drop_graph() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -309,6 +326,7 @@ struct kst_backend {
         a.pools[1] = pools[1].p;
         a.cap = slots;
         a.bank_len = bank_len;
+        a.tap_bytes = 2;
         a.clamp = bank_clamp;
         a.bank = bank.p;
         a.params = params.p;
@@ -361,6 +379,7 @@ struct kst_backend {
         a.phase_shift = sort.phase_shift;
         a.phase_mask = sort.phase_mask;
         a.dead_region = sort.dead_region;
+        a.fused = fast ? 1u : 0u;
         a.sel_cnt = sel_cnt.p;
         a.sel_min = sel_min.p;
         a.sel_max = sel_max.p;
@@ -760,6 +779,7 @@ int kst_render_collect(kst_backend *b, float *out, size_t n) {
     const BlockResult &r = *b->pin_result;
     b->live = (uint32_t)r.active_voices;
     b->parity ^= 1;
+    b->perm_valid = b->fast;
     float pk;
     std::memcpy(&pk, &r.peak_bits, 4);
     b->peak = pk;
@@ -894,6 +914,7 @@ int kst_run_staged(kst_backend *b, int use_graph) {
     HIPCHK(hipMemcpy(&r, b->g_results.p + (b->staged - 1), sizeof r, hipMemcpyDeviceToHost));
     b->live = (uint32_t)r.active_voices;
     if (b->staged & 1) b->parity ^= 1;
+    b->perm_valid = b->fast;
     b->stolen = r.stolen;
     b->dropped = r.dropped;
     std::memcpy(&b->peak, &r.peak_bits, 4);
@@ -1025,8 +1046,10 @@ int kst_batch_run(kst_batch *bt, int use_graph) {
         kst_backend *b = order[i];
         if (!b->staged) return fail(KST_ESTATE, "a batch member has no staged blocks");
         HIPCHK(hipStreamSynchronize(b->stream));
+        if (int rc = b->settle()) return rc;
         const uint32_t n = b->slots;
         host[i] = b->args(b->render_groups_for(n));
+        host[i].fused = 0;  // a batch compacts with its own gather pass
         steps = std::max(steps, b->staged);
         sel = std::max(sel, std::max(1u, std::min(std::min(b->cus, SEL_MAX_BLOCKS),
                                                   (b->cfg.max_voices + SEL_THREADS * 4 - 1) /
@@ -1116,6 +1139,7 @@ int64_t kst_pool_export(kst_backend *b, uint32_t *soa, size_t stride) {
     if (!b || !soa) return fail(KST_EINVAL, "null argument");
     if (stride < b->live) return fail(KST_EINVAL, "stride %zu below the live count %u", stride, b->live);
     if (hipSetDevice(b->device) != hipSuccess) return fail(KST_EDEVICE, "hipSetDevice");
+    if (int rc = b->settle()) return rc;
     for (uint32_t f = 0; f < VOICE_FIELDS; ++f) {
         if (b->live == 0) break;
         hipError_t e = hipMemcpy(soa + (size_t)f * stride, b->pools[b->parity].p + (size_t)f * b->slots,
@@ -1139,8 +1163,10 @@ int kst_pool_import(kst_backend *b, const uint32_t *soa, size_t n, size_t stride
     HIPCHK(hipMemcpy(&s, b->st.p, sizeof s, hipMemcpyDeviceToHost));
     s.live = (uint32_t)n;
     s.parity = b->parity;
+    s.perm_valid = 0;  // the imported pool is in order
     HIPCHK(hipMemcpy(b->st.p, &s, sizeof s, hipMemcpyHostToDevice));
     b->live = (uint32_t)n;
+    b->perm_valid = false;
     return KST_OK;
 }
 

@@ -100,7 +100,7 @@ struct DevState {
     uint32_t sel_count;             // remaining candidates
     uint32_t sel_done;              // candidates few enough to rank in LDS
     uint32_t cand_n;                // candidates collected
-    uint32_t sel_pad;
+    uint32_t perm_valid;            // the live pool is read through the last sort's permutation
 };
 
 // Per-block result row, what stats() reads after each render.
@@ -115,6 +115,7 @@ struct Args {
     uint32_t cap;  // allocated slots = SoA stride (Config::pool_slots)
     uint32_t bank_len;
     uint32_t clamp;  // the bank holds -32768, so taps need the -1 clamp
+    uint32_t tap_bytes;  // 2: bytes per int16 frame (a runtime value, so tap offsets use one mad_u24)
     const int16_t *bank;
     const kst_region_params *params;
     uint32_t ppv;           // params per variant
@@ -149,6 +150,10 @@ struct Args {
     uint32_t *sort_tot;   // [passes][radix] digit totals
     uint32_t sort_on, sort_passes, digit_bits, tiles_max;
     uint32_t region_shift, stage_shift, phase_shift, phase_mask, dead_region;
+    // Fused compaction (wave64 per-block and staged runs): the render reads
+    // the survivors through the previous sort's permutation and writes every
+    // field of every voice to the other pool buffer, so no gather pass runs.
+    uint32_t fused;
     uint32_t *sel_cnt;             // [SEL_MAX_BLOCKS][SEL_BINS] per-workgroup counts
     unsigned long long *sel_min;   // [SEL_MAX_BLOCKS][SEL_BINS]
     unsigned long long *sel_max;   // [SEL_MAX_BLOCKS][SEL_BINS]
@@ -163,12 +168,21 @@ __device__ __forceinline__ T *sel2(T *const (&pair)[2], uint32_t i) {
     return i ? pair[1] : pair[0];
 }
 
+// The live pool's voice i sits at slot perm[i] after a fused block (the last
+// sort's permutation, never applied as a copy), at slot i otherwise.
+__device__ __forceinline__ const uint32_t *live_perm(const Args &a) {
+    return a.st->perm_valid ? sel2(a.idx, a.sort_passes & 1) : nullptr;
+}
+
 // ---- launchers (kst_pool.hip) ----
 void launch_begin(const Args &a, hipStream_t s);
 void launch_select(const Args &a, uint32_t grid, uint32_t rounds, hipStream_t s);
 void launch_spawn(const Args &a, uint32_t grid, hipStream_t s);
 void launch_sort(const Args &a, uint32_t grid, uint32_t gather_grid, hipStream_t s);
 void launch_end(const Args &a, hipStream_t s);
+// Apply a fused block's permutation as a copy (before the pool is read or
+// written in place): gather, then flip to the gathered buffer.
+void launch_settle(const Args &a, uint32_t gather_grid, hipStream_t s);
 
 // ---- launchers (kst_render.hip) ----
 // Returns false when the wave64 form does not cover this configuration.

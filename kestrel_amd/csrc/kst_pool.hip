@@ -188,6 +188,7 @@ __device__ __forceinline__ void sel_hist_body(const Args &a) {
     }
     __syncthreads();
     const uint32_t *pool = sel2(a.pools, st->parity);
+    const uint32_t *perm = live_perm(a);
     const uint32_t live = st->live;
     const uint32_t lim = (live + 63) & ~63u;  // whole waves, so ballots see every lane
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < lim;
@@ -196,7 +197,7 @@ __device__ __forceinline__ void sel_hist_body(const Args &a) {
         unsigned long long key = 0;
         uint32_t d = 0;
         if (i < live) {
-            key = steal_key(a, pool, i);
+            key = steal_key(a, pool, perm ? perm[i] : i);
             match = sel_match(key, prefix, top);
             d = (uint32_t)(key >> lo) & dmask;
         }
@@ -307,6 +308,7 @@ __device__ __forceinline__ void sel_collect_body(const Args &a) {
     if (st->steal_k == 0) return;
     __shared__ uint32_t s_n, s_base;
     uint32_t *pool = sel2(a.pools, st->parity);
+    const uint32_t *perm = live_perm(a);
     const uint32_t live = st->live;
     const uint32_t top = st->sel_top;
     const unsigned long long prefix = st->sel_prefix;
@@ -315,8 +317,9 @@ __device__ __forceinline__ void sel_collect_body(const Args &a) {
     __syncthreads();
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < live;
          i += gridDim.x * blockDim.x) {
-        const unsigned long long kb = top >= 64 ? 0ull : steal_key(a, pool, i) >> top;
-        if (kb < pb) mark_victim(a, pool, i);
+        const uint32_t slot = perm ? perm[i] : i;
+        const unsigned long long kb = top >= 64 ? 0ull : steal_key(a, pool, slot) >> top;
+        if (kb < pb) mark_victim(a, pool, slot);
         else if (kb == pb) atomicAdd(&s_n, 1u);
     }
     __syncthreads();
@@ -326,11 +329,12 @@ __device__ __forceinline__ void sel_collect_body(const Args &a) {
     __syncthreads();
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < live;
          i += gridDim.x * blockDim.x) {
-        const unsigned long long key = steal_key(a, pool, i);
+        const uint32_t slot = perm ? perm[i] : i;
+        const unsigned long long key = steal_key(a, pool, slot);
         if ((top >= 64 ? 0ull : key >> top) == pb) {
             const uint32_t at = s_base + atomicAdd(&s_n, 1u);
             a.cand_key[at] = key;
-            a.cand_slot[at] = i;
+            a.cand_slot[at] = slot;
         }
     }
 }
@@ -418,7 +422,8 @@ __device__ __forceinline__ void spawn_body(const Args &a) {
     const bool thin = a.admit_rule == ADMIT_EVEN && n < want;
     const uint32_t base = st->live;
     const kst_spawn_cmd *cmds = a.cmds + st->cmd_off;
-    uint32_t *pool = sel2(a.pools, st->parity);
+    // A fused block renders into the other buffer; its spawns go there directly.
+    uint32_t *pool = sel2(a.pools, st->parity ^ a.fused);
     const uint32_t c = a.cap;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += gridDim.x * blockDim.x) {
@@ -687,21 +692,62 @@ __global__ void __launch_bounds__(256) kb_gather(const Args *__restrict__ aa) {
     gather_body(aa[blockIdx.z]);
 }
 
+// After a fused block the survivors are a prefix of the sorted keys (dead
+// voices sort after every survivor), so their count is the first dead
+// position: one workgroup finds the chunk holding it, then counts the
+// survivors inside that chunk. Two rounds of reads, no global atomics.
+__device__ uint32_t count_survivors(const Args &a, uint32_t n) {
+    __shared__ uint32_t s_chunk, s_cnt;
+    const uint32_t *skey = sel2(a.keys, a.sort_passes & 1);
+    const uint32_t t = threadIdx.x, T = blockDim.x;
+    if (t == 0) {
+        s_chunk = T;
+        s_cnt = 0;
+    }
+    __syncthreads();
+    const uint32_t b0 = (uint32_t)((uint64_t)t * n / T);
+    const uint32_t e0 = (uint32_t)((uint64_t)(t + 1) * n / T);
+    if (e0 > b0 && !key_alive(a, skey[e0 - 1])) atomicMin(&s_chunk, t);
+    __syncthreads();
+    const uint32_t c = s_chunk;
+    if (c == T) return n;  // every key alive (the same branch in every thread)
+    const uint32_t lo = (uint32_t)((uint64_t)c * n / T);
+    const uint32_t hi = (uint32_t)((uint64_t)(c + 1) * n / T);
+    uint32_t k = 0;
+    for (uint32_t p = lo + t; p < hi; p += T) k += key_alive(a, skey[p]) ? 1u : 0u;
+    if (k) atomicAdd(&s_cnt, k);
+    __syncthreads();
+    return lo + s_cnt;
+}
+
 __device__ __forceinline__ void end_body(const Args &a) {
     DevState *st = a.st;
     if (st->idle) return;
+    const uint32_t live_new = a.fused ? count_survivors(a, st->live_total) : st->live_new;
+    if (threadIdx.x != 0) return;
     BlockResult r;
-    r.active_voices = st->live_new;
+    r.active_voices = live_new;
     r.stolen = st->stolen;
     r.dropped = st->dropped;
     r.peak_bits = st->peak_bits;
     r.pad = 0;
     a.results[st->block] = r;
-    st->live = st->live_new;
+    st->live_new = live_new;
+    st->live = live_new;
     st->parity ^= 1;
+    st->perm_valid = a.fused;
     st->block += 1;
 }
 __global__ void k_end(Args a) { end_body(a); }
+
+// After the gather of a settle: the gathered buffer becomes the live pool, in
+// order, with no permutation pending.
+__global__ void k_settle(Args a) {
+    DevState *st = a.st;
+    st->live = st->live_new;
+    st->parity ^= 1;
+    st->perm_valid = 0;
+}
 // The same pass over a batch of renders: blockIdx.z picks the render.
 __global__ void kb_end(const Args *__restrict__ aa) {
     end_body(aa[blockIdx.z]);
@@ -731,10 +777,15 @@ void launch_sort(const Args &a, uint32_t grid, uint32_t gather_grid, hipStream_t
         k_sort_scan<<<radix, 256, 0, s>>>(a, p);
         k_sort_scatter<<<grid, SORT_TPB, 0, s>>>(a, p);
     }
-    k_gather<<<gather_grid, 256, 0, s>>>(a);
+    if (!a.fused) k_gather<<<gather_grid, 256, 0, s>>>(a);
 }
 
-void launch_end(const Args &a, hipStream_t s) { k_end<<<1, 1, 0, s>>>(a); }
+void launch_end(const Args &a, hipStream_t s) { k_end<<<1, a.fused ? 256 : 1, 0, s>>>(a); }
+
+void launch_settle(const Args &a, uint32_t gather_grid, hipStream_t s) {
+    k_gather<<<gather_grid, 256, 0, s>>>(a);
+    k_settle<<<1, 1, 0, s>>>(a);
+}
 
 void launch_batch_begin(const Args *aa, uint32_t nr, hipStream_t s) {
     kb_begin<<<dim3(1, 1, nr), 256, 0, s>>>(aa);

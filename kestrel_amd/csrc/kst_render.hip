@@ -95,6 +95,28 @@ __device__ __forceinline__ void load_voice(const Args &a, const uint32_t *pool, 
     v.p = load_params(a, v.variant, v.params_base);
 }
 
+// Fused compaction: the fields a block never changes move with the voice to
+// its slot in the other buffer (store_voice writes the rest).
+__device__ __forceinline__ void copy_static(const Args &a, const uint32_t *src, uint32_t s,
+                                            uint32_t *dst, uint32_t d, const Voice &v) {
+    const uint32_t c = a.cap;
+    dst[F_STEP_LO * c + d] = (uint32_t)v.base_step;
+    dst[F_STEP_HI * c + d] = (uint32_t)(v.base_step >> 32);
+    dst[F_SMP_BASE * c + d] = v.base;
+    dst[F_SMP_LEN * c + d] = v.len;
+    dst[F_LOOP_START * c + d] = v.ls;
+    dst[F_LOOP_END * c + d] = v.le;
+    dst[F_FLAGS * c + d] = v.vflags;
+    dst[F_GAIN_L * c + d] = __float_as_uint(v.bgl);
+    dst[F_GAIN_R * c + d] = __float_as_uint(v.bgr);
+    dst[F_PARAMS * c + d] = v.params_base;
+    dst[F_REGION * c + d] = v.region;
+    dst[F_GATE_SLOT * c + d] = v.gate_slot;
+    dst[F_ORDINAL * c + d] = v.ordinal;
+    dst[F_NOTE_LO * c + d] = src[F_NOTE_LO * c + s];
+    dst[F_NOTE_HI * c + d] = src[F_NOTE_HI * c + s];
+}
+
 // Write back what the block changed, clear the per-block fields
 // (src/cpu.rs:605-612), and emit this voice's sort key (shaders/sort.wgsl:
 // 246-262) or its dead bit when sorting is off.
@@ -379,7 +401,10 @@ __device__ __forceinline__ uint64_t fast_limit(const Args &a, const Voice &v) {
     // address the voice struct in scratch memory.
     const uint32_t pick = 0u - (uint32_t)looping;
     const uint64_t limit = v.len ^ ((v.le ^ v.len) & pick);
-    const uint64_t l1 = limit > reach ? limit - reach : 0;
+    // The in-sample index also stays below 2^24, so the tap offset is one
+    // 24-bit multiply-add (longer samples take the edge path there).
+    const uint64_t lc = limit < (1u << 24) ? limit : (1u << 24);
+    const uint64_t l1 = lc > reach ? lc - reach : 0;
     const uint64_t room = (uint64_t)a.bank_len;
     const uint64_t l2 = room > (uint64_t)v.base + reach ? room - v.base - reach : 0;
     return (l1 < l2 ? l1 : l2) << 32;  // < 2^63: a phase at or past 2^63 never passes
@@ -397,11 +422,22 @@ __device__ __forceinline__ bool fast_ok(const Voice &v, uint64_t lim, uint32_t f
     return v.phase < lim && v.step * SUBTILE < lim - v.phase;
 }
 
+// The bank as a raw buffer resource: a tap load takes its 32-bit byte offset
+// straight from a VGPR (no 64-bit address arithmetic per load), and a load
+// past the padded bank reads 0 instead of faulting. The host keeps the pool
+// under 2^31 frames, so every byte offset fits 32 bits. Dword 3 = 0x00020000:
+// 32-bit data format, the CDNA3/CDNA4 untyped-buffer setting.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t bank_rsrc(const Args &a) {
+    const uint64_t nb = ((uint64_t)a.bank_len + 8) * 2;  // the pool plus its guard frames
+    const uint32_t n = nb > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)nb;
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<int16_t *>(a.bank), (short)0, (int)n,
+                                             0x00020000);
+}
+
 // One 32-bit word holding the int16 frames at byte offsets off and off + 2
-// (gfx950 global loads take 2-byte-aligned addresses).
-__device__ __forceinline__ uint32_t tap_pair(const char *bank, uint32_t byte_off) {
-    typedef uint32_t u32_a2 __attribute__((aligned(2)));
-    return *reinterpret_cast<const u32_a2 *>(bank + byte_off);
+// (gfx950 buffer loads take 2-byte-aligned offsets).
+__device__ __forceinline__ uint32_t tap_pair(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
+    return __builtin_amdgcn_raw_buffer_load_b32(r, byte_off, 0, 0);
 }
 
 // int16 -> f32 as src/cpu.rs:190-194 does it: times 1/32767, clamped at -1.
@@ -427,24 +463,27 @@ __device__ __forceinline__ void fast_subtile(const Args &a, Voice &v, float *tb,
         v.phase = (uint64_t)(INTERP == 2 ? 1u : 0u) << 32;
         v.base = 0;
     }
-    const char *bank = reinterpret_cast<const char *>(a.bank);
+    const __amdgpu_buffer_rsrc_t bank = bank_rsrc(a);
     // Every frame's taps first: the phase chain is integer arithmetic, so all
     // of the sub-tile's loads can be in flight before the first is needed.
     uint32_t w0[SUBTILE], w1[INTERP == 2 ? SUBTILE : 1];
     {
-        // base + idx in the high word: one 64-bit add and one shift per frame
-        // (no carry out: the straight path keeps base + idx + reach in the bank).
-        uint64_t at = v.phase + ((uint64_t)v.base << 32);
+        // Byte offset 2 * (base + idx) as one 24-bit multiply-add of the
+        // phase's high word (idx < 2^24 on the straight path, which also keeps
+        // base + idx + reach in the bank). The phase chain is the one the
+        // frame loop below runs, so the compiler shares it.
+        const uint32_t base2 = v.base * 2u - (INTERP == 2 ? 2u : 0u);
+        uint64_t ph = v.phase;
 #pragma unroll
         for (uint32_t i = 0; i < SUBTILE; ++i) {
-            const uint32_t off = (uint32_t)(at >> 32) * 2u;
+            const uint32_t off = __umul24((uint32_t)(ph >> 32), a.tap_bytes) + base2;
             if (INTERP == 2) {
-                w0[i] = tap_pair(bank, off - 2);  // frames idx-1, idx
-                w1[i] = tap_pair(bank, off + 2);  // frames idx+1, idx+2
+                w0[i] = tap_pair(bank, off);      // frames idx-1, idx
+                w1[i] = tap_pair(bank, off + 4);  // frames idx+1, idx+2
             } else {
                 w0[i] = tap_pair(bank, off);      // frames idx, idx+1
             }
-            at += v.step;
+            ph += v.step;
         }
     }
     float m, c, thi, tlo;
@@ -452,6 +491,9 @@ __device__ __forceinline__ void fast_subtile(const Args &a, Voice &v, float *tb,
     const bool filt = FILT == 1 || (FILT == 2 && (v.p.flags & RP_FILTER));
     float g = fminf(v.level, 1.0f);
     bool kill = false;
+    // Cubic keeps twice the taps in flight: its frame loop runs a 32-bit
+    // chain of the phase's low word instead of sharing the tap loop's.
+    uint32_t lo32 = (uint32_t)v.phase;
 #pragma unroll
     for (uint32_t i = 0; i < SUBTILE; ++i) {
         if (!ENV_STATIC) {
@@ -471,7 +513,7 @@ __device__ __forceinline__ void fast_subtile(const Args &a, Voice &v, float *tb,
             }
             g = fminf(v.level, 1.0f);
         }
-        const float flo = (float)(uint32_t)v.phase;
+        const float flo = (float)(INTERP == 2 ? lo32 : (uint32_t)v.phase);
         float s;
         if (INTERP == 0) {
             s = norm_lo<CLAMP>(w0[i]);
@@ -502,8 +544,12 @@ __device__ __forceinline__ void fast_subtile(const Args &a, Voice &v, float *tb,
         }
         tb[(2 * i) * TROW + lane] = y * v.gl;
         tb[(2 * i + 1) * TROW + lane] = y * v.gr;
-        v.phase += v.step;
+        if (INTERP == 2)
+            lo32 += (uint32_t)v.step;
+        else
+            v.phase += v.step;
     }
+    if (INTERP == 2) v.phase += v.step * SUBTILE;  // the same sum, mod 2^64
     if (!ENV_STATIC && kill && v.stage == ENV_DEAD) v.gl = v.gr = 0.0f;
 }
 
@@ -572,7 +618,7 @@ __device__ __forceinline__ void fast_subtile_edge(const Args &a, Voice &v, float
     const bool lur = (v.vflags & VF_LOOP_UNTIL_RELEASE) != 0;
     const bool looping = sounding && loop_en && !(lur && v.stage >= ENV_RELEASE);
     const uint64_t span_hi = (uint64_t)(v.le - v.ls) << 32;
-    const char *bank = reinterpret_cast<const char *>(a.bank);
+    const __amdgpu_buffer_rsrc_t bank = bank_rsrc(a);
     const uint32_t base2 = v.base * 2u;
     // Frames [f0 + i] that sound: started, not past the sample, not past the fade.
     uint32_t amask = 0;
@@ -703,8 +749,18 @@ __device__ __forceinline__ void render_fast_body(const Args &a) {
     const uint32_t half = lane >> 5;  // which 32 voices this lane sums over
     const uint32_t o = lane & 31;     // which output of the sub-tile it owns
     const uint32_t vi = batch * 64 + lane;
+    const bool in = vi < live_total;
+    // Fused: voice vi of this block is survivor perm[vi] of the live pool
+    // (vi < live) or this block's spawn, which k_spawn wrote to the other
+    // buffer at vi; either way the voice leaves in the other buffer at vi.
+    uint32_t *dst = a.fused ? sel2(a.pools, st->parity ^ 1) : pool;
+    const bool survivor = !a.fused || vi < st->live;
+    const uint32_t *perm = a.fused ? live_perm(a) : nullptr;
+    const uint32_t *src = survivor ? pool : dst;
+    const uint32_t slot = (survivor && perm && in) ? perm[vi] : vi;
     Voice v;
-    load_voice(a, pool, vi, vi < live_total, v);
+    load_voice(a, src, slot, in, v);
+    if (a.fused && in && survivor) copy_static(a, src, slot, dst, vi, v);
     uint64_t lim = fast_limit<INTERP>(a, v);
     for (uint32_t s = 0; s < F / SUBTILE; ++s) {
         const uint32_t f0 = s * SUBTILE;
@@ -744,13 +800,16 @@ __device__ __forceinline__ void render_fast_body(const Args &a) {
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    if (v.live) store_voice(a, pool, vi, v);
+    if (v.live) store_voice(a, dst, vi, v);
 }
+// At most 128 VGPRs: four waves per SIMD, the occupancy the kernel is tuned for.
 template <int INTERP, bool CHAN, bool CLAMP>
-__global__ void __launch_bounds__(256) k_render_fast(Args a) { render_fast_body<INTERP, CHAN, CLAMP>(a); }
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+k_render_fast(Args a) { render_fast_body<INTERP, CHAN, CLAMP>(a); }
 // The same pass over a batch of renders: blockIdx.z picks the render.
 template <int INTERP, bool CHAN, bool CLAMP>
-__global__ void __launch_bounds__(256) kb_render_fast(const Args *__restrict__ aa) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+kb_render_fast(const Args *__restrict__ aa) {
     render_fast_body<INTERP, CHAN, CLAMP>(aa[blockIdx.z]);
 }
 
