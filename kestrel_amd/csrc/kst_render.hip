@@ -412,11 +412,12 @@ __device__ __forceinline__ uint64_t fast_limit(const Args &a, const Voice &v) {
 
 // Can this lane take the straight-line path for frames [f0, f0 + SUBTILE)?
 // Silent lanes (no voice, or DEAD) always can: they are neutralised.
+// A steal fade in the sub-tile is checked apart (fade_in): the straight path
+// has a variant for it.
 template <int INTERP>
 __device__ __forceinline__ bool fast_ok(const Voice &v, uint64_t lim, uint32_t f0) {
     if (!v.live || v.stage == ENV_DEAD) return true;
     if (v.start_rel > f0) return false;
-    if (v.stop_rel != 0 && v.stop_rel <= f0 + SUBTILE) return false;
     if (INTERP == 2 && (v.phase >> 32) == 0) return false;  // tap idx-1
     // phase + SUBTILE * step < lim, without wrapping (step < 2^58).
     return v.phase < lim && v.step * SUBTILE < lim - v.phase;
@@ -453,10 +454,20 @@ __device__ __forceinline__ float norm_hi(uint32_t w) {
     return CLAMP ? fmaxf(f, -1.0f) : f;
 }
 
+// Does this lane's steal fade (src/cpu.rs:577-586) reach frames [f0, f0 + SUBTILE)?
+__device__ __forceinline__ bool fade_in(const Voice &v, uint32_t f0) {
+    return v.live && v.stage != ENV_DEAD && v.stop_rel != 0 && v.stop_rel <= f0 + SUBTILE;
+}
+
 // FILT: 0 every sounding lane unfiltered, 1 every sounding lane filtered,
 // 2 mixed. ENV_STATIC: every sounding lane sustains through the sub-tile.
-template <int INTERP, bool CLAMP, int FILT, bool ENV_STATIC>
-__device__ __forceinline__ void fast_subtile(const Args &a, Voice &v, float *tb, uint32_t lane) {
+// FADE: some lane's steal fade reaches the sub-tile. Frame f of a lane with
+// d = f + 1 - stop_rel >= 0 is scaled by the fade table's 1 - d / fade, and
+// from d == fade on it is silent and the voice dies -- what the edge path and
+// the per-frame path do, so all three give the same bits.
+template <int INTERP, bool CLAMP, int FILT, bool ENV_STATIC, bool FADE>
+__device__ __forceinline__ void fast_subtile(const Args &a, Voice &v, float *tb,
+                                             const float *ftab, uint32_t lane, uint32_t f0) {
     if (!v.live || v.stage == ENV_DEAD) {
         v.gl = v.gr = 0.0f;
         v.step = 0;
@@ -491,6 +502,10 @@ __device__ __forceinline__ void fast_subtile(const Args &a, Voice &v, float *tb,
     const bool filt = FILT == 1 || (FILT == 2 && (v.p.flags & RP_FILTER));
     float g = fminf(v.level, 1.0f);
     bool kill = false;
+    const bool fades = FADE && v.stop_rel != 0;
+    int32_t d = (int32_t)(f0 + 1) - (int32_t)v.stop_rel;
+    const int32_t fade = (int32_t)a.fade;
+    bool died = false;
     // Cubic keeps twice the taps in flight: its frame loop runs a 32-bit
     // chain of the phase's low word instead of sharing the tap loop's.
     uint32_t lo32 = (uint32_t)v.phase;
@@ -542,8 +557,19 @@ __device__ __forceinline__ void fast_subtile(const Args &a, Voice &v, float *tb,
             v.z1 = v.p.b1 * x - v.p.a1 * y + v.z2;
             v.z2 = b0x - v.p.a2 * y;
         }
-        tb[(2 * i) * TROW + lane] = y * v.gl;
-        tb[(2 * i + 1) * TROW + lane] = y * v.gr;
+        if (FADE) {
+            const bool fading = fades && d >= 0;
+            const bool gone = fading && d >= fade;
+            const float fg = ftab[d < 0 ? 0 : (d < fade ? d : fade - 1)];
+            y = (fading && !gone) ? y * fg : y;
+            tb[(2 * i) * TROW + lane] = gone ? 0.0f : y * v.gl;
+            tb[(2 * i + 1) * TROW + lane] = gone ? 0.0f : y * v.gr;
+            died |= gone;
+            ++d;
+        } else {
+            tb[(2 * i) * TROW + lane] = y * v.gl;
+            tb[(2 * i + 1) * TROW + lane] = y * v.gr;
+        }
         if (INTERP == 2)
             lo32 += (uint32_t)v.step;
         else
@@ -551,23 +577,29 @@ __device__ __forceinline__ void fast_subtile(const Args &a, Voice &v, float *tb,
     }
     if (INTERP == 2) v.phase += v.step * SUBTILE;  // the same sum, mod 2^64
     if (!ENV_STATIC && kill && v.stage == ENV_DEAD) v.gl = v.gr = 0.0f;
+    if (FADE && died) {
+        v.stage = ENV_DEAD;
+        v.level = 0.0f;
+        v.gl = v.gr = 0.0f;
+    }
 }
 
-template <int INTERP, bool CLAMP>
+template <int INTERP, bool CLAMP, bool FADE>
 __device__ __forceinline__ void fast_subtile_dispatch(const Args &a, Voice &v, float *tb,
-                                                      uint32_t lane) {
+                                                      const float *ftab, uint32_t lane,
+                                                      uint32_t f0) {
     const bool sounding = v.live && v.stage != ENV_DEAD;
-    const bool env_static = !__any(sounding && v.stage != ENV_SUSTAIN);
+    const bool env_static = !FADE && !__any(sounding && v.stage != ENV_SUSTAIN);
     const bool f = (v.p.flags & RP_FILTER) != 0;
     if (!__any(sounding && !f)) {
         if (env_static)
-            fast_subtile<INTERP, CLAMP, 1, true>(a, v, tb, lane);
+            fast_subtile<INTERP, CLAMP, 1, true, false>(a, v, tb, ftab, lane, f0);
         else
-            fast_subtile<INTERP, CLAMP, 1, false>(a, v, tb, lane);
+            fast_subtile<INTERP, CLAMP, 1, false, FADE>(a, v, tb, ftab, lane, f0);
     } else if (!__any(sounding && f)) {
-        fast_subtile<INTERP, CLAMP, 0, false>(a, v, tb, lane);
+        fast_subtile<INTERP, CLAMP, 0, false, FADE>(a, v, tb, ftab, lane, f0);
     } else {
-        fast_subtile<INTERP, CLAMP, 2, false>(a, v, tb, lane);
+        fast_subtile<INTERP, CLAMP, 2, false, FADE>(a, v, tb, ftab, lane, f0);
     }
 }
 
@@ -767,8 +799,12 @@ __device__ __forceinline__ void render_fast_body(const Args &a) {
         if (f0 % a.gate_frames == 0 &&
             gate_tile<CHAN>(a, gates, chan, chan_flags, f0 / a.gate_frames, v))
             lim = fast_limit<INTERP>(a, v);
-        if (!__any(!fast_ok<INTERP>(v, lim, f0))) {
-            fast_subtile_dispatch<INTERP, CLAMP>(a, v, tb, lane);
+        const bool straight = !__any(!fast_ok<INTERP>(v, lim, f0));
+        const bool fading = __any(fade_in(v, f0));
+        if (straight && !fading) {
+            fast_subtile_dispatch<INTERP, CLAMP, false>(a, v, tb, ftab, lane, f0);
+        } else if (straight && a.fade_lds != 0) {
+            fast_subtile_dispatch<INTERP, CLAMP, true>(a, v, tb, ftab, lane, f0);
         } else if (INTERP != 2 && !__any(!edge_ok<INTERP>(a, v, f0))) {
             fast_subtile_edge<INTERP, CLAMP>(a, v, tb, ftab, lane, f0);
         } else {
