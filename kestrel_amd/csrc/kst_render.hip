@@ -465,7 +465,11 @@ __device__ __forceinline__ bool fade_in(const Voice &v, uint32_t f0) {
 // d = f + 1 - stop_rel >= 0 is scaled by the fade table's 1 - d / fade, and
 // from d == fade on it is silent and the voice dies -- what the edge path and
 // the per-frame path do, so all three give the same bits.
-template <int INTERP, bool CLAMP, int FILT, bool ENV_STATIC, bool FADE>
+// WRAP (linear and nearest): some looping lane reaches its loop end in the
+// sub-tile. Every frame's phase wraps back by one loop span once its index
+// reaches the loop end, and a linear tap after the loop end is the loop start
+// (src/cpu.rs:243-259), as the edge path does it.
+template <int INTERP, bool CLAMP, int FILT, bool ENV_STATIC, bool FADE, bool WRAP = false>
 __device__ __forceinline__ void fast_subtile(const Args &a, Voice &v, float *tb,
                                              const float *ftab, uint32_t lane, uint32_t f0) {
     if (!v.live || v.stage == ENV_DEAD) {
@@ -478,7 +482,27 @@ __device__ __forceinline__ void fast_subtile(const Args &a, Voice &v, float *tb,
     // Every frame's taps first: the phase chain is integer arithmetic, so all
     // of the sub-tile's loads can be in flight before the first is needed.
     uint32_t w0[SUBTILE], w1[INTERP == 2 ? SUBTILE : 1];
-    {
+    uint64_t wrapped_end = 0;
+    if (WRAP) {
+        const bool sounding = v.live && v.stage != ENV_DEAD;
+        const bool looping = sounding && (v.vflags & VF_LOOP) != 0 &&
+                             !((v.vflags & VF_LOOP_UNTIL_RELEASE) != 0 && v.stage >= ENV_RELEASE);
+        const uint32_t le = looping ? v.le : 0xFFFFFFFFu;  // never reached when not looping
+        const uint32_t span = v.le - v.ls;
+        const uint32_t ls_tap = looping ? (uint32_t)(uint16_t)a.bank[v.base + v.ls] << 16 : 0u;
+        const uint32_t base2 = v.base * 2u;
+        uint64_t ph = v.phase;
+#pragma unroll
+        for (uint32_t i = 0; i < SUBTILE; ++i) {
+            const uint32_t idx = (uint32_t)(ph >> 32);
+            uint32_t w = tap_pair(bank, __umul24(idx, a.tap_bytes) + base2);
+            if (INTERP == 1) w = idx + 1 == le ? (w & 0xFFFFu) | ls_tap : w;
+            w0[i] = w;
+            ph += v.step;
+            if ((uint32_t)(ph >> 32) >= le) ph -= (uint64_t)span << 32;
+        }
+        wrapped_end = ph;
+    } else {
         // Byte offset 2 * (base + idx) as one 24-bit multiply-add of the
         // phase's high word (idx < 2^24 on the straight path, which also keeps
         // base + idx + reach in the bank). The phase chain is the one the
@@ -528,7 +552,7 @@ __device__ __forceinline__ void fast_subtile(const Args &a, Voice &v, float *tb,
             }
             g = fminf(v.level, 1.0f);
         }
-        const float flo = (float)(INTERP == 2 ? lo32 : (uint32_t)v.phase);
+        const float flo = (float)(INTERP == 2 || WRAP ? lo32 : (uint32_t)v.phase);
         float s;
         if (INTERP == 0) {
             s = norm_lo<CLAMP>(w0[i]);
@@ -570,12 +594,15 @@ __device__ __forceinline__ void fast_subtile(const Args &a, Voice &v, float *tb,
             tb[(2 * i) * TROW + lane] = y * v.gl;
             tb[(2 * i + 1) * TROW + lane] = y * v.gr;
         }
-        if (INTERP == 2)
+        if (INTERP == 2 || WRAP)
             lo32 += (uint32_t)v.step;
         else
             v.phase += v.step;
     }
-    if (INTERP == 2) v.phase += v.step * SUBTILE;  // the same sum, mod 2^64
+    if (WRAP)
+        v.phase = wrapped_end;
+    else if (INTERP == 2)
+        v.phase += v.step * SUBTILE;  // the same sum, mod 2^64
     if (!ENV_STATIC && kill && v.stage == ENV_DEAD) v.gl = v.gr = 0.0f;
     if (FADE && died) {
         v.stage = ENV_DEAD;
@@ -633,6 +660,22 @@ __device__ __forceinline__ bool edge_ok(const Args &a, const Voice &v, uint32_t 
     // A fade this sub-tile needs the staged table.
     if (v.stop_rel != 0 && v.stop_rel <= f0 + SUBTILE && a.fade_lds == 0) return false;
     return true;
+}
+
+// Can this lane take the straight path's WRAP variant for frames [f0, f0 +
+// SUBTILE)? A started, looping lane whose loop is longer than its step (one
+// wrap per frame at most), with the tap offsets inside the 24-bit range and the
+// bank. Other lanes of the wave must pass fast_ok.
+__device__ __forceinline__ bool wrap_ok(const Args &a, const Voice &v, uint32_t f0) {
+    if (!v.live || v.stage == ENV_DEAD) return true;
+    if (v.start_rel > f0) return false;
+    const bool looping = (v.vflags & VF_LOOP) != 0 &&
+                         !((v.vflags & VF_LOOP_UNTIL_RELEASE) != 0 && v.stage >= ENV_RELEASE);
+    if (!looping) return false;
+    const uint32_t hi = (uint32_t)(v.phase >> 32);
+    if (v.ls >= v.le || hi >= v.le || (uint32_t)(v.step >> 32) >= v.le - v.ls) return false;
+    if (v.le >= (1u << 24)) return false;
+    return (uint64_t)v.base + v.le < a.bank_len;
 }
 
 template <int INTERP, bool CLAMP>
@@ -799,12 +842,15 @@ __device__ __forceinline__ void render_fast_body(const Args &a) {
         if (f0 % a.gate_frames == 0 &&
             gate_tile<CHAN>(a, gates, chan, chan_flags, f0 / a.gate_frames, v))
             lim = fast_limit<INTERP>(a, v);
-        const bool straight = !__any(!fast_ok<INTERP>(v, lim, f0));
+        const bool fok = fast_ok<INTERP>(v, lim, f0);
+        const bool straight = !__any(!fok);
         const bool fading = __any(fade_in(v, f0));
         if (straight && !fading) {
             fast_subtile_dispatch<INTERP, CLAMP, false>(a, v, tb, ftab, lane, f0);
         } else if (straight && a.fade_lds != 0) {
             fast_subtile_dispatch<INTERP, CLAMP, true>(a, v, tb, ftab, lane, f0);
+        } else if (INTERP != 2 && !fading && !__any(!(fok || wrap_ok(a, v, f0)))) {
+            fast_subtile<INTERP, CLAMP, 2, false, false, true>(a, v, tb, ftab, lane, f0);
         } else if (INTERP != 2 && !__any(!edge_ok<INTERP>(a, v, f0))) {
             fast_subtile_edge<INTERP, CLAMP>(a, v, tb, ftab, lane, f0);
         } else {
