@@ -84,6 +84,18 @@ __device__ __forceinline__ uint32_t xcd_group(uint32_t id, uint32_t n) {
     return x * q + (x < r ? x : r) + k;
 }
 
+// The same runs, each walked from its end: the first workgroups an XCD
+// launches take the last batches of its run. The pool's tail holds the block's
+// new voices (late starts, the slowest waves), so they start first instead of
+// trailing the launch.
+__device__ __forceinline__ uint32_t xcd_group_rev(uint32_t id, uint32_t n) {
+    constexpr uint32_t XCDS = 8;
+    const uint32_t x = id % XCDS, k = id / XCDS;
+    const uint32_t q = n / XCDS, r = n % XCDS;
+    const uint32_t cnt = q + (x < r ? 1u : 0u);
+    return x * q + (x < r ? x : r) + (cnt - 1 - k);
+}
+
 struct DevState {
     uint32_t live;    // compacted live count at the start of the block
     uint32_t parity;  // which pool buffer holds the live pool

@@ -812,7 +812,7 @@ __device__ __forceinline__ void render_fast_body(const Args &a) {
     const uint32_t live_total = st->live_total;
     const uint32_t groups = (live_total + 64 * wpg - 1) / (64 * wpg);
     if (blockIdx.x >= groups) return;
-    const uint32_t batch = xcd_group(blockIdx.x, groups) * wpg + wave;
+    const uint32_t batch = xcd_group_rev(blockIdx.x, groups) * wpg + wave;
     if (batch * 64 >= live_total) return;
     const uint32_t b = st->block;
     const uint32_t chan_flags = CHAN ? st->chan_flags : 0u;
