@@ -170,6 +170,7 @@ struct kst_backend {
     uint32_t max_steal = 0;
     SortPlan sort{};
     uint32_t sort_passes = 1, digit_bits = 1, tiles_max = 1;
+    uint32_t dead_digit = ~0u;  // Args::dead_digit
     uint32_t max_groups = 1;  // render workgroups at capacity (plain: partial-buffer width)
     uint32_t max_rows = 1;    // wave64 partial rows at capacity (one per 64-voice batch)
 
@@ -380,6 +381,7 @@ struct kst_backend {
         a.phase_mask = sort.phase_mask;
         a.dead_region = sort.dead_region;
         a.fused = fast ? 1u : 0u;
+        a.dead_digit = dead_digit;
         a.sel_cnt = sel_cnt.p;
         a.sel_min = sel_min.p;
         a.sel_max = sel_max.p;
@@ -519,6 +521,13 @@ int kst_create(const kst_config *cfg, const int16_t *pool, size_t pool_len,
         b->sort_passes = 1;
     }
     b->tiles_max = (b->slots + SORT_TILE - 1) / SORT_TILE;
+    {
+        // Dead keys are those at or above D (the dead region in the key's top
+        // field, or the dead bit); the last pass sees key >> s.
+        const uint64_t D = c.sort_voices ? (uint64_t)b->sort.dead_region << b->sort.region_shift : 1;
+        const uint32_t sl = (b->sort_passes - 1) * b->digit_bits;
+        b->dead_digit = (D & ((1ull << sl) - 1)) == 0 ? (uint32_t)(D >> sl) : ~0u;
+    }
     // A fourth global round keeps the final single-workgroup pass short once
     // the pool is large enough that three leave tens of thousands of keys.
     b->sel_rounds = c.max_voices > (1u << 22) ? 4 : 3;

@@ -43,13 +43,14 @@ constexpr uint32_t ADMIT_EVEN = 1;
 // and largest key of every bin, so the next digit starts at the highest bit
 // that still varies inside the chosen bin (note-id keys skip their zero high
 // bits). Once the bin holds at most SEL_CAP keys, one workgroup finishes the
-// rounds on the collected candidates. Per-workgroup histograms are stored without atomics and summed by one
+// rounds on the collected candidates, held in registers (16 per thread).
+// Per-workgroup histograms are stored without atomics and summed by one
 // workgroup: the same global address hit from every CU is what costs.
 constexpr uint32_t SEL_BITS = 8;
 constexpr uint32_t SEL_BINS = 1u << SEL_BITS;
 constexpr uint32_t SEL_MAX_BLOCKS = 1024;
-constexpr uint32_t SEL_CAP = 4096;
 constexpr uint32_t SEL_THREADS = 1024;
+constexpr uint32_t SEL_CAP = 16 * SEL_THREADS;
 
 // Locality sort tiles: 256 threads x 8 items, processed in index order.
 constexpr uint32_t SORT_TPB = 256;
@@ -166,6 +167,10 @@ struct Args {
     // the survivors through the previous sort's permutation and writes every
     // field of every voice to the other pool buffer, so no gather pass runs.
     uint32_t fused;
+    // Fused: the top digit of the last sort pass at which dead keys begin, when
+    // that boundary falls on a digit (the survivor count is then that digit's
+    // base); ~0u when it does not.
+    uint32_t dead_digit;
     uint32_t *sel_cnt;             // [SEL_MAX_BLOCKS][SEL_BINS] per-workgroup counts
     unsigned long long *sel_min;   // [SEL_MAX_BLOCKS][SEL_BINS]
     unsigned long long *sel_max;   // [SEL_MAX_BLOCKS][SEL_BINS]

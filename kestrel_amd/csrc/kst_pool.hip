@@ -289,6 +289,7 @@ __device__ __forceinline__ void sel_refine_body(const Args &a, uint32_t blocks) 
         st->sel_done = n <= SEL_CAP || top == 0;
     }
 }
+
 __global__ void __launch_bounds__(SEL_THREADS) k_sel_refine(Args a, uint32_t blocks) { sel_refine_body(a, blocks); }
 // The same pass over a batch of renders: blockIdx.z picks the render.
 __global__ void __launch_bounds__(SEL_THREADS) kb_sel_refine(const Args *__restrict__ aa, uint32_t blocks) {
@@ -344,12 +345,105 @@ __global__ void __launch_bounds__(SEL_THREADS) kb_sel_collect(const Args *__rest
     sel_collect_body(aa[blockIdx.z]);
 }
 
+// The final rounds when the candidates fit in registers: at most SEL_CAP keys
+// (16 per thread) that agree on every bit from 32 up, so only their low words
+// are held. Each round histograms the active keys' next digit in LDS, one wave
+// scans the bins and picks the one holding the k-th key, keys in lower bins
+// become victims and keys in higher bins drop out -- the choices of the
+// general rounds below, without a trip through global memory per round.
+__device__ void sel_final_regs(const Args &a, uint32_t n) {
+    DevState *st = a.st;
+    constexpr uint32_t PER = SEL_CAP / SEL_THREADS;
+    __shared__ uint32_t cnt[SEL_BINS], mn[SEL_BINS], mx[SEL_BINS];
+    __shared__ uint32_t s_bin, s_excl;
+    uint32_t *pool = sel2(a.pools, st->parity);
+    const uint32_t t = threadIdx.x;
+    // Key j of this thread is candidate j * SEL_THREADS + t; its slot is read
+    // back only when it turns out a victim.
+    uint32_t key[PER];
+    uint32_t act = 0;  // bit j: key[j] is still a candidate
+#pragma unroll
+    for (uint32_t j = 0; j < PER; ++j) {
+        const uint32_t i = j * SEL_THREADS + t;
+        const bool in = i < n;
+        act |= in ? 1u << j : 0u;
+        key[j] = in ? (uint32_t)a.cand_key[i] : 0u;
+    }
+    uint32_t top = st->sel_top, k = st->sel_k, count = n;
+    // Each round lowers top by at least SEL_BITS; unique keys leave one.
+    for (uint32_t round = 0; round <= 32 / SEL_BITS && count > 1; ++round) {
+        const uint32_t lo = sel_lo(top);
+        const uint32_t dmask = (1u << (top - lo)) - 1;
+        if (t < SEL_BINS) {
+            cnt[t] = 0;
+            mn[t] = ~0u;
+            mx[t] = 0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t j = 0; j < PER; ++j) {
+            if (!((act >> j) & 1u)) continue;
+            const uint32_t d = (key[j] >> lo) & dmask;
+            atomicAdd(&cnt[d], 1u);
+            atomicMin(&mn[d], key[j]);
+            atomicMax(&mx[d], key[j]);
+        }
+        __syncthreads();
+        if (t < 64) {
+            // Four bins per lane, an inclusive scan across the wave.
+            uint32_t c[4], sum = 0;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                c[b] = cnt[4 * t + b];
+                sum += c[b];
+            }
+            uint32_t incl = sum;
+            for (uint32_t off = 1; off < 64; off <<= 1) {
+                const uint32_t y = __shfl_up(incl, off);
+                if (t >= off) incl += y;
+            }
+            uint32_t run = incl - sum;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                if (run < k && run + c[b] >= k) {
+                    s_bin = 4 * t + b;
+                    s_excl = run;
+                }
+                run += c[b];
+            }
+        }
+        __syncthreads();
+        const uint32_t bin = s_bin;
+        k -= s_excl;
+        count = cnt[bin];
+        const uint32_t diff = mn[bin] ^ mx[bin];
+#pragma unroll
+        for (uint32_t j = 0; j < PER; ++j) {
+            if (!((act >> j) & 1u)) continue;
+            const uint32_t d = (key[j] >> lo) & dmask;
+            if (d < bin) mark_victim(a, pool, a.cand_slot[j * SEL_THREADS + t]);
+            if (d != bin) act &= ~(1u << j);
+        }
+        top = diff ? 32 - __clz(diff) : 0;
+        __syncthreads();  // every read of this round's bins before the next clears them
+    }
+    // The one key left is the k-th smallest.
+#pragma unroll
+    for (uint32_t j = 0; j < PER; ++j)
+        if (((act >> j) & 1u) && count == 1 && k == 1)
+            mark_victim(a, pool, a.cand_slot[j * SEL_THREADS + t]);
+}
+
 // Finish on the candidates with more rounds in one workgroup until the bin
 // holds the k-th key alone; keys below each chosen bin become victims as the
 // candidate list is narrowed, and the last one standing is the k-th.
 __device__ __forceinline__ void sel_final_body(const Args &a) {
     DevState *st = a.st;
     if (st->steal_k == 0) return;
+    if (st->cand_n <= SEL_CAP && st->sel_top <= 32 && blockDim.x == SEL_THREADS) {
+        sel_final_regs(a, st->cand_n);
+        return;
+    }
     __shared__ uint32_t cnt[SEL_BINS], scan[SEL_BINS];
     __shared__ unsigned long long mn[SEL_BINS], mx[SEL_BINS];
     __shared__ uint32_t s_n;
@@ -603,6 +697,11 @@ __device__ __forceinline__ void sort_scatter_body(const Args &a, uint32_t p) {
     for (uint32_t i = threadIdx.x; i < WAVES * SORT_MAX_RADIX; i += SORT_TPB)
         wc[i / SORT_MAX_RADIX][i % SORT_MAX_RADIX] = 0;
     __syncthreads();
+    // Fused: survivors sort before every dead key, so when the dead keys begin
+    // at a digit boundary of the last pass, their count is that digit's base.
+    if (a.fused && p + 1 == a.sort_passes && a.dead_digit != ~0u && blockIdx.x == 0 &&
+        threadIdx.x == 0)
+        st->live_new = a.dead_digit < radix ? base[a.dead_digit] : n;
     for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
         uint32_t kk[SORT_IPT], ii[SORT_IPT];
 #pragma unroll
@@ -723,7 +822,8 @@ __device__ uint32_t count_survivors(const Args &a, uint32_t n) {
 __device__ __forceinline__ void end_body(const Args &a) {
     DevState *st = a.st;
     if (st->idle) return;
-    const uint32_t live_new = a.fused ? count_survivors(a, st->live_total) : st->live_new;
+    const uint32_t live_new = a.fused && a.dead_digit == ~0u ? count_survivors(a, st->live_total)
+                                                           : st->live_new;
     if (threadIdx.x != 0) return;
     BlockResult r;
     r.active_voices = live_new;
