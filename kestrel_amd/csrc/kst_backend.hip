@@ -181,7 +181,7 @@ struct kst_backend {
     DevBuf<kst_region_params> params;
     DevBuf<float> partials, rsum, fade_tab;
     uint32_t fade_lds = 0;
-    DevBuf<uint32_t> keys[2], idx[2], sort_hist, sort_tot, sel_cnt, cand_slot;
+    DevBuf<uint32_t> keys[2], idx[2], sort_hist, sort_tot, sel_cnt, cand_slot, alive_rows;
     DevBuf<unsigned long long> cand_key, sel_min, sel_max;
     DevBuf<DevState> st;
 
@@ -223,6 +223,10 @@ struct kst_backend {
 
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    // Staged runs select the next block's victims on a side stream while the
+    // current block reduces and sorts (default priority, like the main stream).
+    hipStream_t side = nullptr;
+    hipEvent_t ev_rendered = nullptr, ev_selected = nullptr;
 
     ~kst_backend() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -244,6 +248,7 @@ struct kst_backend {
         sel_min.release();
         sel_max.release();
         cand_slot.release();
+        alive_rows.release();
         cand_key.release();
         st.release();
         g_gates.release();
@@ -258,6 +263,9 @@ struct kst_backend {
         if (pin_cmds) (void)hipHostFree(pin_cmds);
         if (pin_out) (void)hipHostFree(pin_out);
         if (pin_result) (void)hipHostFree(pin_result);
+        if (ev_rendered) (void)hipEventDestroy(ev_rendered);
+        if (ev_selected) (void)hipEventDestroy(ev_selected);
+        if (side) (void)hipStreamDestroy(side);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -387,24 +395,35 @@ struct kst_backend {
         a.sel_max = sel_max.p;
         a.cand_key = cand_key.p;
         a.cand_slot = cand_slot.p;
+        a.alive_rows = alive_rows.p;
         return a;
+    }
+
+    // Victim selection for the block about to run, or (ahead) for the block
+    // after the last render.
+    void launch_sel(uint32_t n_live, bool ahead, hipStream_t s) {
+        Args a = args(1);
+        a.sel_ahead = ahead ? 1u : 0u;
+        launch_select(a, std::max(1u, std::min(std::min(cus, SEL_MAX_BLOCKS),
+                                               (n_live + SEL_THREADS * 4 - 1) / (SEL_THREADS * 4))),
+                      sel_rounds, s);
     }
 
     // The per-block chain. `n_live`/`n_total` size the grids; the kernels read
     // the true counts from the device state, so capacity-sized grids (staged
-    // runs) are as correct as exact ones (per-block calls).
+    // runs) are as correct as exact ones (per-block calls). Pipelined (staged
+    // runs, wave64 form): this block's victims were selected during the last
+    // block; after the render the side stream selects the next block's while
+    // this one reduces, sorts and compacts, and the block ends by joining it.
     void launch_block(uint32_t n_live, uint32_t n_total, bool may_steal, bool spawn, bool chan,
-                      hipEvent_t *ev) {
+                      hipEvent_t *ev, bool pipelined = false) {
         const bool timed = ev != nullptr;
         const uint32_t rg = render_groups_for(n_total);
         Args a = args(rg);
         hipStream_t s = stream;
         if (timed) (void)hipEventRecord(ev[0], s);
+        if (may_steal && !pipelined) launch_sel(n_live, false, s);
         launch_begin(a, s);
-        if (may_steal)
-            launch_select(a, std::max(1u, std::min(std::min(cus, SEL_MAX_BLOCKS),
-                                                   (n_live + SEL_THREADS * 4 - 1) / (SEL_THREADS * 4))),
-                          sel_rounds, s);
         if (timed) (void)hipEventRecord(ev[1], s);
         if (spawn) launch_spawn(a, per_voice_grid(n_total), s);
         if (timed) (void)hipEventRecord(ev[2], s);
@@ -413,6 +432,12 @@ struct kst_backend {
         else
             launch_render_plain(a, chan, s);
         if (timed) (void)hipEventRecord(ev[3], s);
+        if (pipelined) {
+            (void)hipEventRecord(ev_rendered, s);
+            (void)hipStreamWaitEvent(side, ev_rendered, 0);
+            launch_sel(n_live, true, side);
+            (void)hipEventRecord(ev_selected, side);
+        }
         if (fast)
             launch_reduce_fast(a, cfg.compensated != 0, (n_total + 63) / 64, s);
         else
@@ -420,6 +445,7 @@ struct kst_backend {
         if (timed) (void)hipEventRecord(ev[4], s);
         launch_sort(a, sort_grid(n_total), per_voice_grid(n_total), s);
         launch_end(a, s);
+        if (pipelined) (void)hipStreamWaitEvent(s, ev_selected, 0);
         if (timed) (void)hipEventRecord(ev[5], s);
     }
 
@@ -501,8 +527,11 @@ int kst_create(const kst_config *cfg, const int16_t *pool, size_t pool_len,
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device) == hipSuccess &&
         cus > 0)
         b->cus = (uint32_t)cus;
-    if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess)
-        return cleanup(fail(KST_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(e)));
+    if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&b->ev_rendered, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&b->ev_selected, hipEventDisableTiming)) != hipSuccess)
+        return cleanup(fail(KST_EDEVICE, "creating streams and events: %s", hipGetErrorString(e)));
 
     const kst_config &c = b->cfg;
     b->fast = c.form == 1 && fast_render_supported(c.block_frames, c.gate_frames, c.interpolation);
@@ -592,6 +621,7 @@ int kst_create(const kst_config *cfg, const int16_t *pool, size_t pool_len,
         b->sel_min.alloc(SEL_MAX_BLOCKS * SEL_BINS) != hipSuccess ||
         b->sel_max.alloc(SEL_MAX_BLOCKS * SEL_BINS) != hipSuccess ||
         b->cand_key.alloc(b->slots) != hipSuccess || b->cand_slot.alloc(b->slots) != hipSuccess ||
+        b->alive_rows.alloc((b->slots + 63) / 64) != hipSuccess ||
         b->st.alloc(1) != hipSuccess)
         return cleanup(fail(KST_EDEVICE, "allocating the select/sort scratch"));
     // The steal fade's gain for d = 0 .. fade-1 (src/cpu.rs:577-586): the same
@@ -890,6 +920,13 @@ int kst_run_staged(kst_backend *b, int use_graph) {
     // Capacity-sized grids: the live count is on the device only.
     const uint32_t n = b->slots;
     const bool chan = b->staged_chan;
+    // The wave64 form selects each block's victims during the block before;
+    // the first block's are selected here.
+    const bool pipe = b->fast;
+    if (pipe) {
+        b->launch_sel(b->cfg.max_voices, false, s);
+        HIPCHK(hipGetLastError());
+    }
     if (use_graph) {
         if (b->graph_exec && b->graph_chan != chan) {
             (void)hipGraphExecDestroy(b->graph_exec);
@@ -899,7 +936,7 @@ int kst_run_staged(kst_backend *b, int use_graph) {
         }
         if (!b->graph_exec) {
             HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            b->launch_block(b->cfg.max_voices, n, true, true, chan, nullptr);
+            b->launch_block(b->cfg.max_voices, n, true, true, chan, nullptr, pipe);
             b->graph_chan = chan;
             hipError_t e = hipStreamEndCapture(s, &b->graph);
             if (e != hipSuccess) return fail(KST_EDEVICE, "graph capture: %s", hipGetErrorString(e));
@@ -913,7 +950,7 @@ int kst_run_staged(kst_backend *b, int use_graph) {
         }
         for (uint32_t i = 0; i < b->staged; ++i)
             b->launch_block(b->cfg.max_voices, n, true, true, chan,
-                            b->profile ? &b->stage_ev[6 * i] : nullptr);
+                            b->profile ? &b->stage_ev[6 * i] : nullptr, pipe);
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipStreamSynchronize(s));

@@ -114,6 +114,14 @@ struct DevState {
     uint32_t sel_done;              // candidates few enough to rank in LDS
     uint32_t cand_n;                // candidates collected
     uint32_t perm_valid;            // the live pool is read through the last sort's permutation
+    // What the last render left, for the next block's victim selection: the
+    // voices it ran (slots [0, snap_n) of pool snap_parity) and its block
+    // index; each of its waves stores how many of its 64 still sound in
+    // Args::alive_rows.
+    uint32_t snap_n, snap_parity, snap_block;
+    // The selection's own view (k_sel_begin): candidates are the sounding
+    // voices among slots [0, sel_n) of pool sel_parity; sel_steal become victims.
+    uint32_t sel_n, sel_parity, sel_steal;
 };
 
 // Per-block result row, what stats() reads after each render.
@@ -167,6 +175,10 @@ struct Args {
     // the survivors through the previous sort's permutation and writes every
     // field of every voice to the other pool buffer, so no gather pass runs.
     uint32_t fused;
+    // Select the victims of the block after the last render (staged runs,
+    // on the side stream) rather than of the block about to run.
+    uint32_t sel_ahead;
+    uint32_t *alive_rows;  // [rows]: sounding voices per 64-voice batch of the last render
     // Fused: the top digit of the last sort pass at which dead keys begin, when
     // that boundary falls on a digit (the survivor count is then that digit's
     // base); ~0u when it does not.

@@ -67,17 +67,73 @@ __device__ __forceinline__ void begin_body(const Args &a) {
     st->peak_bits = 0;
     st->stolen += steal_k;
     st->dropped += raw - spawn;
-    st->sel_prefix = 0;
-    st->sel_top = 64;
-    st->sel_k = steal_k > 0 ? (uint32_t)steal_k : 1;
-    st->sel_count = (uint32_t)live;
-    st->sel_done = live <= SEL_CAP;  // few enough for the final workgroup alone
-    st->cand_n = 0;
 }
 __global__ void k_begin(Args a) { begin_body(a); }
 // The same pass over a batch of renders: blockIdx.z picks the render.
 __global__ void kb_begin(const Args *__restrict__ aa) {
     begin_body(aa[blockIdx.z]);
+}
+
+// The selection's set-up (src/cpu.rs:345-371 picks victims among the voices
+// live at the block's start). Candidates are the sounding voices among a
+// range of slots: after a fused block, the slots its render wrote (survivors
+// and the dying, in render order -- the victim set does not depend on order);
+// otherwise the compacted live pool. With sel_ahead the block is the one after
+// the last render, selected while that render's block still reduces and
+// sorts; the steal count repeats k_begin's arithmetic from the same inputs.
+__device__ __forceinline__ void sel_begin_body(const Args &a) {
+    DevState *st = a.st;
+    const bool from_render = a.sel_ahead || (a.fused && st->perm_valid);
+    uint32_t blk, n, par, live;
+    if (from_render) {
+        blk = a.sel_ahead ? st->snap_block + 1 : st->block;
+        n = st->snap_n;
+        par = a.sel_ahead ? st->snap_parity : st->parity;
+        // The render's per-wave counts of sounding voices, summed here.
+        __shared__ uint32_t s_live;
+        if (threadIdx.x == 0) s_live = 0;
+        __syncthreads();
+        const uint32_t rows = (n + 63) / 64;
+        uint32_t c = 0;
+        for (uint32_t r = threadIdx.x; r < rows; r += blockDim.x) c += a.alive_rows[r];
+        for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+        if ((threadIdx.x & 63) == 0 && c) atomicAdd(&s_live, c);
+        __syncthreads();
+        live = s_live;
+    } else {
+        blk = st->block;
+        n = st->live;
+        par = st->parity;
+        live = st->live;
+    }
+    if (threadIdx.x != 0) return;
+    uint64_t steal_k = 0;
+    if (blk < a.nblocks) {
+        const uint64_t cap = a.max_voices, raw = a.cmd_counts[blk];
+        const uint64_t want = raw < cap ? raw : cap;
+        if ((uint64_t)live + want > cap && a.steal_rule != STEAL_DROP_NEW) {
+            steal_k = live + want - cap;
+            if (steal_k > live) steal_k = live;
+            if (steal_k > a.max_steal) steal_k = a.max_steal;
+        }
+    }
+    st->sel_n = n;
+    st->sel_parity = par;
+    st->sel_steal = (uint32_t)steal_k;
+    st->sel_prefix = 0;
+    st->sel_top = 64;
+    st->sel_k = steal_k > 0 ? (uint32_t)steal_k : 1;
+    st->sel_count = live;
+    st->sel_done = live <= SEL_CAP;  // few enough for the final workgroup alone
+    st->cand_n = 0;
+}
+__global__ void __launch_bounds__(1024) k_sel_begin(Args a) { sel_begin_body(a); }
+__global__ void __launch_bounds__(1024) kb_sel_begin(const Args *__restrict__ aa) {
+    sel_begin_body(aa[blockIdx.z]);
+}
+
+__device__ __forceinline__ bool sounding_at(const Args &a, const uint32_t *pool, uint32_t i) {
+    return pool[F_ENV_STAGE * a.cap + i] != ENV_DEAD;
 }
 
 // ---------------------------------------------------------------------------
@@ -174,7 +230,7 @@ __device__ void sel_choose(DevState *st, const uint32_t *cnt,
 // candidates' next digit, stored without atomics.
 __device__ __forceinline__ void sel_hist_body(const Args &a) {
     DevState *st = a.st;
-    if (st->steal_k == 0 || st->sel_done) return;
+    if (st->sel_steal == 0 || st->sel_done) return;
     __shared__ uint32_t cnt[SEL_BINS];
     __shared__ unsigned long long mn[SEL_BINS], mx[SEL_BINS];
     const uint32_t top = st->sel_top;
@@ -187,17 +243,16 @@ __device__ __forceinline__ void sel_hist_body(const Args &a) {
         mx[i] = 0;
     }
     __syncthreads();
-    const uint32_t *pool = sel2(a.pools, st->parity);
-    const uint32_t *perm = live_perm(a);
-    const uint32_t live = st->live;
-    const uint32_t lim = (live + 63) & ~63u;  // whole waves, so ballots see every lane
+    const uint32_t *pool = sel2(a.pools, st->sel_parity);
+    const uint32_t n = st->sel_n;
+    const uint32_t lim = (n + 63) & ~63u;  // whole waves, so ballots see every lane
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < lim;
          i += gridDim.x * blockDim.x) {
         bool match = false;
         unsigned long long key = 0;
         uint32_t d = 0;
-        if (i < live) {
-            key = steal_key(a, pool, perm ? perm[i] : i);
+        if (i < n && sounding_at(a, pool, i)) {
+            key = steal_key(a, pool, i);
             match = sel_match(key, prefix, top);
             d = (uint32_t)(key >> lo) & dmask;
         }
@@ -222,7 +277,7 @@ __global__ void __launch_bounds__(SEL_THREADS) kb_sel_hist(const Args *__restric
 // largest keys are then read back from every row.
 __device__ __forceinline__ void sel_refine_body(const Args &a, uint32_t blocks) {
     DevState *st = a.st;
-    if (st->steal_k == 0 || st->sel_done) return;
+    if (st->sel_steal == 0 || st->sel_done) return;
     constexpr uint32_t Q = SEL_THREADS / SEL_BINS;
     constexpr uint32_t WAVES = SEL_THREADS / 64;
     __shared__ uint32_t cnt[Q][SEL_BINS], scan[SEL_BINS];
@@ -306,21 +361,20 @@ __device__ __forceinline__ void mark_victim(const Args &a, uint32_t *pool, uint3
 // (one global reservation per workgroup).
 __device__ __forceinline__ void sel_collect_body(const Args &a) {
     DevState *st = a.st;
-    if (st->steal_k == 0) return;
+    if (st->sel_steal == 0) return;
     __shared__ uint32_t s_n, s_base;
-    uint32_t *pool = sel2(a.pools, st->parity);
-    const uint32_t *perm = live_perm(a);
-    const uint32_t live = st->live;
+    uint32_t *pool = sel2(a.pools, st->sel_parity);
+    const uint32_t n = st->sel_n;
     const uint32_t top = st->sel_top;
     const unsigned long long prefix = st->sel_prefix;
     const unsigned long long pb = top >= 64 ? 0ull : prefix >> top;
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < live;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += gridDim.x * blockDim.x) {
-        const uint32_t slot = perm ? perm[i] : i;
-        const unsigned long long kb = top >= 64 ? 0ull : steal_key(a, pool, slot) >> top;
-        if (kb < pb) mark_victim(a, pool, slot);
+        if (!sounding_at(a, pool, i)) continue;
+        const unsigned long long kb = top >= 64 ? 0ull : steal_key(a, pool, i) >> top;
+        if (kb < pb) mark_victim(a, pool, i);
         else if (kb == pb) atomicAdd(&s_n, 1u);
     }
     __syncthreads();
@@ -328,14 +382,14 @@ __device__ __forceinline__ void sel_collect_body(const Args &a) {
     __syncthreads();
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < live;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += gridDim.x * blockDim.x) {
-        const uint32_t slot = perm ? perm[i] : i;
-        const unsigned long long key = steal_key(a, pool, slot);
+        if (!sounding_at(a, pool, i)) continue;
+        const unsigned long long key = steal_key(a, pool, i);
         if ((top >= 64 ? 0ull : key >> top) == pb) {
             const uint32_t at = s_base + atomicAdd(&s_n, 1u);
             a.cand_key[at] = key;
-            a.cand_slot[at] = slot;
+            a.cand_slot[at] = i;
         }
     }
 }
@@ -356,7 +410,7 @@ __device__ void sel_final_regs(const Args &a, uint32_t n) {
     constexpr uint32_t PER = SEL_CAP / SEL_THREADS;
     __shared__ uint32_t cnt[SEL_BINS], mn[SEL_BINS], mx[SEL_BINS];
     __shared__ uint32_t s_bin, s_excl;
-    uint32_t *pool = sel2(a.pools, st->parity);
+    uint32_t *pool = sel2(a.pools, st->sel_parity);
     const uint32_t t = threadIdx.x;
     // Key j of this thread is candidate j * SEL_THREADS + t; its slot is read
     // back only when it turns out a victim.
@@ -439,7 +493,7 @@ __device__ void sel_final_regs(const Args &a, uint32_t n) {
 // candidate list is narrowed, and the last one standing is the k-th.
 __device__ __forceinline__ void sel_final_body(const Args &a) {
     DevState *st = a.st;
-    if (st->steal_k == 0) return;
+    if (st->sel_steal == 0) return;
     if (st->cand_n <= SEL_CAP && st->sel_top <= 32 && blockDim.x == SEL_THREADS) {
         sel_final_regs(a, st->cand_n);
         return;
@@ -447,7 +501,7 @@ __device__ __forceinline__ void sel_final_body(const Args &a) {
     __shared__ uint32_t cnt[SEL_BINS], scan[SEL_BINS];
     __shared__ unsigned long long mn[SEL_BINS], mx[SEL_BINS];
     __shared__ uint32_t s_n;
-    uint32_t *pool = sel2(a.pools, st->parity);
+    uint32_t *pool = sel2(a.pools, st->sel_parity);
     uint32_t n = st->cand_n;
     // Each round lowers sel_top by at least SEL_BITS (the chosen bin's keys
     // agree on every bit above the digit), and unique keys leave one
@@ -858,6 +912,7 @@ __global__ void kb_end(const Args *__restrict__ aa) {
 void launch_begin(const Args &a, hipStream_t s) { k_begin<<<1, 256, 0, s>>>(a); }
 
 void launch_select(const Args &a, uint32_t grid, uint32_t rounds, hipStream_t s) {
+    k_sel_begin<<<1, 1024, 0, s>>>(a);
     for (uint32_t r = 0; r < rounds; ++r) {
         k_sel_hist<<<grid, SEL_THREADS, 0, s>>>(a);
         k_sel_refine<<<1, SEL_THREADS, 0, s>>>(a, grid);
@@ -893,6 +948,7 @@ void launch_batch_begin(const Args *aa, uint32_t nr, hipStream_t s) {
 
 void launch_batch_select(const Args *aa, uint32_t nr, uint32_t grid, uint32_t rounds,
                          hipStream_t s) {
+    kb_sel_begin<<<dim3(1, 1, nr), 1024, 0, s>>>(aa);
     for (uint32_t r = 0; r < rounds; ++r) {
         kb_sel_hist<<<dim3(grid, 1, nr), SEL_THREADS, 0, s>>>(aa);
         kb_sel_refine<<<dim3(1, 1, nr), SEL_THREADS, 0, s>>>(aa, grid);

@@ -810,6 +810,12 @@ __device__ __forceinline__ void render_fast_body(const Args &a) {
 
     DevState *st = a.st;
     const uint32_t live_total = st->live_total;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        // What the next block's victim selection reads: the slots written here.
+        st->snap_n = live_total;
+        st->snap_parity = a.fused ? st->parity ^ 1 : st->parity;
+        st->snap_block = st->block;
+    }
     const uint32_t groups = (live_total + 64 * wpg - 1) / (64 * wpg);
     if (blockIdx.x >= groups) return;
     const uint32_t batch = xcd_group_rev(blockIdx.x, groups) * wpg + wave;
@@ -883,6 +889,8 @@ __device__ __forceinline__ void render_fast_body(const Args &a) {
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     if (v.live) store_voice(a, dst, vi, v);
+    const uint64_t sounding = __ballot(v.live && v.stage != ENV_DEAD);
+    if (lane == 0) a.alive_rows[batch] = (uint32_t)__popcll(sounding);
 }
 // At most 128 VGPRs: four waves per SIMD, the occupancy the kernel is tuned for.
 template <int INTERP, bool CHAN, bool CLAMP>
