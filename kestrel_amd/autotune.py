@@ -19,6 +19,7 @@ from . import workload as W
 from .config import Config
 
 CANDIDATES = [(f, w) for f in (256, 512, 1024) for w in (1, 2)]
+MIXED_WPG_MARGIN = 1.03
 
 
 def size_class(max_voices: int) -> int:
@@ -76,7 +77,11 @@ def tune(specs: list[dict], log=print) -> dict:
     for cls, interp in classes:
         pool = 1 << cls
         scores = {c: measure(pool, interp, *c) for c in CANDIDATES}
-        pick = max(scores, key=scores.get)
+        # In a batch, members that differ in waves per workgroup need render
+        # launches of their own, run one after the other; two waves per
+        # workgroup has to win clearly to be worth that.
+        eff = {(f, w): v / (MIXED_WPG_MARGIN if w != 1 else 1.0) for (f, w), v in scores.items()}
+        pick = max(eff, key=eff.get)
         best[(cls, interp)] = pick
         log(f"autotune pool 2^{cls} interp {interp}: " +
             ", ".join(f"{f}/{w}: {v:.2e}" for (f, w), v in scores.items()) + f" -> {pick}")

@@ -116,6 +116,10 @@ int validate(const kst_config &c) {
     return KST_OK;
 }
 
+// Pools from this size on select the next block's victims on the side stream
+// during a staged run.
+constexpr uint32_t PIPELINE_MIN_VOICES = 1u << 15;
+
 struct SortPlan {
     uint32_t bits, region_shift, stage_shift, phase_shift, phase_mask, dead_region;
 };
@@ -921,8 +925,9 @@ int kst_run_staged(kst_backend *b, int use_graph) {
     const uint32_t n = b->slots;
     const bool chan = b->staged_chan;
     // The wave64 form selects each block's victims during the block before;
-    // the first block's are selected here.
-    const bool pipe = b->fast;
+    // the first block's are selected here. Small pools keep the plain chain:
+    // their selection is a few launches, cheaper than the fork and join.
+    const bool pipe = b->fast && b->cfg.max_voices >= PIPELINE_MIN_VOICES;
     if (pipe) {
         b->launch_sel(b->cfg.max_voices, false, s);
         HIPCHK(hipGetLastError());

@@ -175,6 +175,46 @@ def test_staged_run_equals_per_block_calls():
         b.close()
 
 
+def test_pipelined_staged_run_equals_per_block_calls_while_stealing():
+    """A staged run of a pool large enough to select each block's victims on
+    the side stream during the block before (with and without the graph)
+    gives the per-block calls' outputs, stats and pool bit for bit, over
+    blocks that steal."""
+    from kestrel_amd.backend import HipSynth
+    cfg = W.config_c2(max_voices=40_000)
+    bank = W.make_bank(11, instruments=16, zones=4, key_lo=0, key_hi=127)
+    notes = W.burst_notes(12, 0.6, base_rate_hz=150_000.0, burst_notes_n=20_000,
+                          burst_every_s=0.25)
+    blocks = [blk for blk in (W.BlockBuilder(cfg, bank, notes).next_block() for _ in range(50))]
+    print("pipelined staged run, 40k voices, 50 blocks", flush=True)
+    a = HipSynth(cfg, bank.pool, bank.params, bank.n_regions, bank.max_sample_len)
+    outs, stats_a = [], []
+    for blk in blocks:
+        a.set_gates(blk.gates)
+        a.set_channels(blk.chans, *blk.flags)
+        a.spawn(blk.cmds)
+        outs.append(a.render().copy())
+        stats_a.append(a.stats())
+    assert stats_a[-1]["stolen"] > 0 and sum(
+        x["stolen"] != y["stolen"] for x, y in zip(stats_a, stats_a[1:])) >= 10
+    pool_a = a.pool_export()
+    for use_graph in (False, True):
+        b = HipSynth(cfg, bank.pool, bank.params, bank.n_regions, bank.max_sample_len)
+        b.stage_blocks(np.concatenate([x.gates for x in blocks]),
+                       np.concatenate([x.chans for x in blocks]),
+                       np.array([x.chan_flag_word for x in blocks], np.uint32),
+                       np.concatenate([x.cmds for x in blocks]),
+                       np.array([len(x.cmds) for x in blocks], np.uint32))
+        b.run_staged(use_graph)
+        out, stats = b.staged_results()
+        assert np.array_equal(out, np.stack(outs)), use_graph
+        assert [s["stolen"] for s in stats] == [s["stolen"] for s in stats_a], use_graph
+        assert [s["active_voices"] for s in stats] == [s["active_voices"] for s in stats_a]
+        assert np.array_equal(b.pool_export(), pool_a), use_graph
+        b.close()
+    a.close()
+
+
 def test_reruns_are_byte_identical():
     cfg, bank, notes = W.workload_c1()
     from kestrel_amd.backend import HipSynth
