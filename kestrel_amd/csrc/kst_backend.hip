@@ -185,7 +185,7 @@ struct kst_backend {
     DevBuf<kst_region_params> params;
     DevBuf<float> partials, rsum, fade_tab;
     uint32_t fade_lds = 0;
-    DevBuf<uint32_t> keys[2], idx[2], sort_hist, sort_tot, sel_cnt, cand_slot, alive_rows;
+    DevBuf<uint32_t> keys[2], idx[2], sort_hist, sort_tot, sel_cnt, cand_slot;
     DevBuf<unsigned long long> cand_key, sel_min, sel_max;
     DevBuf<DevState> st;
 
@@ -252,7 +252,6 @@ struct kst_backend {
         sel_min.release();
         sel_max.release();
         cand_slot.release();
-        alive_rows.release();
         cand_key.release();
         st.release();
         g_gates.release();
@@ -399,7 +398,6 @@ struct kst_backend {
         a.sel_max = sel_max.p;
         a.cand_key = cand_key.p;
         a.cand_slot = cand_slot.p;
-        a.alive_rows = alive_rows.p;
         return a;
     }
 
@@ -625,7 +623,6 @@ int kst_create(const kst_config *cfg, const int16_t *pool, size_t pool_len,
         b->sel_min.alloc(SEL_MAX_BLOCKS * SEL_BINS) != hipSuccess ||
         b->sel_max.alloc(SEL_MAX_BLOCKS * SEL_BINS) != hipSuccess ||
         b->cand_key.alloc(b->slots) != hipSuccess || b->cand_slot.alloc(b->slots) != hipSuccess ||
-        b->alive_rows.alloc((b->slots + 63) / 64) != hipSuccess ||
         b->st.alloc(1) != hipSuccess)
         return cleanup(fail(KST_EDEVICE, "allocating the select/sort scratch"));
     // The steal fade's gain for d = 0 .. fade-1 (src/cpu.rs:577-586): the same

@@ -51,6 +51,8 @@ constexpr uint32_t SEL_BINS = 1u << SEL_BITS;
 constexpr uint32_t SEL_MAX_BLOCKS = 1024;
 constexpr uint32_t SEL_THREADS = 1024;
 constexpr uint32_t SEL_CAP = 16 * SEL_THREADS;
+// DevState::sel_steal before the selection has counted its candidates.
+constexpr uint32_t SEL_PENDING = 0xFFFFFFFFu;
 
 // Locality sort tiles: 256 threads x 8 items, processed in index order.
 constexpr uint32_t SORT_TPB = 256;
@@ -116,12 +118,13 @@ struct DevState {
     uint32_t perm_valid;            // the live pool is read through the last sort's permutation
     // What the last render left, for the next block's victim selection: the
     // voices it ran (slots [0, snap_n) of pool snap_parity) and its block
-    // index; each of its waves stores how many of its 64 still sound in
-    // Args::alive_rows.
+    // index. How many still sound is counted by the selection's first round.
     uint32_t snap_n, snap_parity, snap_block;
     // The selection's own view (k_sel_begin): candidates are the sounding
-    // voices among slots [0, sel_n) of pool sel_parity; sel_steal become victims.
-    uint32_t sel_n, sel_parity, sel_steal;
+    // voices among slots [0, sel_n) of pool sel_parity; sel_steal become
+    // victims (SEL_PENDING until the candidates are counted), of a block
+    // with sel_want note-ons.
+    uint32_t sel_n, sel_parity, sel_steal, sel_want;
 };
 
 // Per-block result row, what stats() reads after each render.
@@ -178,7 +181,6 @@ struct Args {
     // Select the victims of the block after the last render (staged runs,
     // on the side stream) rather than of the block about to run.
     uint32_t sel_ahead;
-    uint32_t *alive_rows;  // [rows]: sounding voices per 64-voice batch of the last render
     // Fused: the top digit of the last sort pass at which dead keys begin, when
     // that boundary falls on a digit (the survivor count is then that digit's
     // base); ~0u when it does not.

@@ -81,56 +81,67 @@ __global__ void kb_begin(const Args *__restrict__ aa) {
 // otherwise the compacted live pool. With sel_ahead the block is the one after
 // the last render, selected while that render's block still reduces and
 // sorts; the steal count repeats k_begin's arithmetic from the same inputs.
+// src/gpu/mod.rs:1015-1040: how many of `live` voices a block with `raw`
+// note-ons steals.
+__device__ __forceinline__ uint32_t steal_count(const Args &a, uint64_t live, uint64_t raw) {
+    const uint64_t cap = a.max_voices;
+    const uint64_t want = raw < cap ? raw : cap;
+    uint64_t steal_k = 0;
+    if (live + want > cap && a.steal_rule != STEAL_DROP_NEW) {
+        steal_k = live + want - cap;
+        if (steal_k > live) steal_k = live;
+        if (steal_k > a.max_steal) steal_k = a.max_steal;
+    }
+    return (uint32_t)steal_k;
+}
+
+// The steal count once the number of sounding candidates is known (the first
+// refine's bin total, or the collected count when no round ran).
+__device__ __forceinline__ uint32_t resolve_steal(const Args &a, DevState *st, uint32_t live) {
+    const uint32_t k = steal_count(a, live, st->sel_want);
+    st->sel_steal = k;
+    st->sel_k = k > 0 ? k : 1;
+    return k;
+}
+
 __device__ __forceinline__ void sel_begin_body(const Args &a) {
+    if (threadIdx.x != 0) return;
     DevState *st = a.st;
     const bool from_render = a.sel_ahead || (a.fused && st->perm_valid);
-    uint32_t blk, n, par, live;
+    uint32_t blk, n, par;
     if (from_render) {
         blk = a.sel_ahead ? st->snap_block + 1 : st->block;
         n = st->snap_n;
         par = a.sel_ahead ? st->snap_parity : st->parity;
-        // The render's per-wave counts of sounding voices, summed here.
-        __shared__ uint32_t s_live;
-        if (threadIdx.x == 0) s_live = 0;
-        __syncthreads();
-        const uint32_t rows = (n + 63) / 64;
-        uint32_t c = 0;
-        for (uint32_t r = threadIdx.x; r < rows; r += blockDim.x) c += a.alive_rows[r];
-        for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
-        if ((threadIdx.x & 63) == 0 && c) atomicAdd(&s_live, c);
-        __syncthreads();
-        live = s_live;
     } else {
         blk = st->block;
         n = st->live;
         par = st->parity;
-        live = st->live;
-    }
-    if (threadIdx.x != 0) return;
-    uint64_t steal_k = 0;
-    if (blk < a.nblocks) {
-        const uint64_t cap = a.max_voices, raw = a.cmd_counts[blk];
-        const uint64_t want = raw < cap ? raw : cap;
-        if ((uint64_t)live + want > cap && a.steal_rule != STEAL_DROP_NEW) {
-            steal_k = live + want - cap;
-            if (steal_k > live) steal_k = live;
-            if (steal_k > a.max_steal) steal_k = a.max_steal;
-        }
     }
     st->sel_n = n;
     st->sel_parity = par;
-    st->sel_steal = (uint32_t)steal_k;
+    st->sel_want = blk < a.nblocks ? a.cmd_counts[blk] : 0u;
     st->sel_prefix = 0;
     st->sel_top = 64;
-    st->sel_k = steal_k > 0 ? (uint32_t)steal_k : 1;
-    st->sel_count = live;
-    st->sel_done = live <= SEL_CAP;  // few enough for the final workgroup alone
     st->cand_n = 0;
+    if (blk >= a.nblocks) {
+        st->sel_steal = 0;
+    } else if (from_render) {
+        // How many of the render's voices still sound is counted by the first
+        // histogram round (or by the collection); the steal count waits for it.
+        st->sel_steal = SEL_PENDING;
+        st->sel_k = 1;
+        st->sel_count = n;
+        st->sel_done = n <= SEL_CAP;
+        return;
+    } else {
+        resolve_steal(a, st, n);
+    }
+    st->sel_count = n;
+    st->sel_done = n <= SEL_CAP;  // few enough for the final workgroup alone
 }
-__global__ void __launch_bounds__(1024) k_sel_begin(Args a) { sel_begin_body(a); }
-__global__ void __launch_bounds__(1024) kb_sel_begin(const Args *__restrict__ aa) {
-    sel_begin_body(aa[blockIdx.z]);
-}
+__global__ void k_sel_begin(Args a) { sel_begin_body(a); }
+__global__ void kb_sel_begin(const Args *__restrict__ aa) { sel_begin_body(aa[blockIdx.z]); }
 
 __device__ __forceinline__ bool sounding_at(const Args &a, const uint32_t *pool, uint32_t i) {
     return pool[F_ENV_STAGE * a.cap + i] != ENV_DEAD;
@@ -278,41 +289,80 @@ __global__ void __launch_bounds__(SEL_THREADS) kb_sel_hist(const Args *__restric
 __device__ __forceinline__ void sel_refine_body(const Args &a, uint32_t blocks) {
     DevState *st = a.st;
     if (st->sel_steal == 0 || st->sel_done) return;
-    constexpr uint32_t Q = SEL_THREADS / SEL_BINS;
+    constexpr uint32_t QUADS = SEL_BINS / 4;          // 16-byte loads per row
+    constexpr uint32_t G = SEL_THREADS / QUADS;       // rows read at once
     constexpr uint32_t WAVES = SEL_THREADS / 64;
-    __shared__ uint32_t cnt[Q][SEL_BINS], scan[SEL_BINS];
+    __shared__ uint32_t part[G][SEL_BINS], cnt[SEL_BINS];
     __shared__ unsigned long long red_lo[WAVES], red_hi[WAVES];
-    __shared__ uint32_t s_bin, s_excl;
-    const uint32_t t = threadIdx.x, bin = t % SEL_BINS, q = t / SEL_BINS;
-    uint32_t c = 0;
+    __shared__ uint32_t s_bin, s_excl, s_k;
+    const uint32_t t = threadIdx.x;
+    // Thread t sums four neighbouring bins over every G-th row: all of its
+    // loads are independent, so the counts arrive in one memory round trip.
+    {
+        const uint32_t b4 = (t % QUADS) * 4, r0 = t / QUADS;
+        uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
 #pragma unroll 8
-    for (uint32_t b = q; b < blocks; b += Q) c += a.sel_cnt[(size_t)b * SEL_BINS + bin];
-    cnt[q][bin] = c;
-    if (t == 0) s_bin = SEL_BINS;
-    __syncthreads();
-    if (q == 0) {
-        for (uint32_t j = 1; j < Q; ++j) c += cnt[j][bin];
-        cnt[0][bin] = c;
-        scan[bin] = c;
+        for (uint32_t r = r0; r < blocks; r += G) {
+            const uint4 x = *reinterpret_cast<const uint4 *>(a.sel_cnt + (size_t)r * SEL_BINS + b4);
+            c0 += x.x;
+            c1 += x.y;
+            c2 += x.z;
+            c3 += x.w;
+        }
+        part[r0][b4] = c0;
+        part[r0][b4 + 1] = c1;
+        part[r0][b4 + 2] = c2;
+        part[r0][b4 + 3] = c3;
     }
     __syncthreads();
-    for (uint32_t off = 1; off < SEL_BINS; off <<= 1) {
-        const uint32_t x = (t < SEL_BINS && t >= off) ? scan[t - off] : 0u;
-        __syncthreads();
-        if (t < SEL_BINS) scan[t] += x;
-        __syncthreads();
-    }
-    const uint32_t k = st->sel_k;
     if (t < SEL_BINS) {
-        const uint32_t excl = scan[t] - cnt[0][t];
-        if (excl < k && excl + cnt[0][t] >= k) {
-            s_bin = t;
-            s_excl = excl;
+        uint32_t c = 0;
+#pragma unroll
+        for (uint32_t g = 0; g < G; ++g) c += part[g][t];
+        cnt[t] = c;
+    }
+    __syncthreads();
+    if (t < 64) {
+        // One wave scans the bins (four per lane) and picks the k-th key's.
+        uint32_t c[4], sum = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            c[b] = cnt[4 * t + b];
+            sum += c[b];
+        }
+        uint32_t incl = sum;
+        for (uint32_t off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(incl, off);
+            if (t >= off) incl += y;
+        }
+        uint32_t k = st->sel_k;
+        if (st->sel_steal == SEL_PENDING) {
+            // The first round over the render's voices: the bins hold every
+            // sounding one, so their total is the live count.
+            const uint32_t total = __shfl(incl, 63);
+            k = steal_count(a, total, st->sel_want);
+            if (t == 0) {
+                st->sel_steal = k;
+                st->sel_k = k > 0 ? k : 1;
+            }
+        }
+        if (t == 0) {
+            s_k = k;
+            s_bin = SEL_BINS;
+        }
+        uint32_t run = incl - sum;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            if (k > 0 && run < k && run + c[b] >= k) {
+                s_bin = 4 * t + b;
+                s_excl = run;
+            }
+            run += c[b];
         }
     }
     __syncthreads();
     const uint32_t chosen = s_bin;
-    if (chosen >= SEL_BINS) return;  // k outside the counts: cannot happen
+    if (chosen >= SEL_BINS) return;  // nothing to steal (or k outside the counts: cannot happen)
     unsigned long long lo = ~0ull, hi = 0;
     for (uint32_t b = t; b < blocks; b += SEL_THREADS) {
         const unsigned long long x = a.sel_min[(size_t)b * SEL_BINS + chosen];
@@ -336,10 +386,10 @@ __device__ __forceinline__ void sel_refine_body(const Args &a, uint32_t blocks) 
         // highest one in which those two differ.
         const unsigned long long diff = lo ^ hi;
         const uint32_t top = diff ? 64 - __clzll(diff) : 0;
-        const uint32_t n = cnt[0][chosen];
+        const uint32_t n = cnt[chosen];
         st->sel_prefix = top >= 64 ? 0ull : (lo >> top) << top;
         st->sel_top = top;
-        st->sel_k = k - s_excl;
+        st->sel_k = s_k - s_excl;
         st->sel_count = n;
         st->sel_done = n <= SEL_CAP || top == 0;
     }
@@ -494,6 +544,16 @@ __device__ void sel_final_regs(const Args &a, uint32_t n) {
 __device__ __forceinline__ void sel_final_body(const Args &a) {
     DevState *st = a.st;
     if (st->sel_steal == 0) return;
+    if (st->sel_steal == SEL_PENDING) {
+        // No round ran: every sounding voice was collected.
+        __syncthreads();  // every thread has read sel_steal before it changes
+        if (threadIdx.x == 0) {
+            resolve_steal(a, st, st->cand_n);
+            st->sel_count = st->cand_n;
+        }
+        __syncthreads();
+        if (st->sel_steal == 0) return;
+    }
     if (st->cand_n <= SEL_CAP && st->sel_top <= 32 && blockDim.x == SEL_THREADS) {
         sel_final_regs(a, st->cand_n);
         return;
@@ -912,7 +972,7 @@ __global__ void kb_end(const Args *__restrict__ aa) {
 void launch_begin(const Args &a, hipStream_t s) { k_begin<<<1, 256, 0, s>>>(a); }
 
 void launch_select(const Args &a, uint32_t grid, uint32_t rounds, hipStream_t s) {
-    k_sel_begin<<<1, 1024, 0, s>>>(a);
+    k_sel_begin<<<1, 1, 0, s>>>(a);
     for (uint32_t r = 0; r < rounds; ++r) {
         k_sel_hist<<<grid, SEL_THREADS, 0, s>>>(a);
         k_sel_refine<<<1, SEL_THREADS, 0, s>>>(a, grid);
@@ -948,7 +1008,7 @@ void launch_batch_begin(const Args *aa, uint32_t nr, hipStream_t s) {
 
 void launch_batch_select(const Args *aa, uint32_t nr, uint32_t grid, uint32_t rounds,
                          hipStream_t s) {
-    kb_sel_begin<<<dim3(1, 1, nr), 1024, 0, s>>>(aa);
+    kb_sel_begin<<<dim3(1, 1, nr), 1, 0, s>>>(aa);
     for (uint32_t r = 0; r < rounds; ++r) {
         kb_sel_hist<<<dim3(grid, 1, nr), SEL_THREADS, 0, s>>>(aa);
         kb_sel_refine<<<dim3(1, 1, nr), SEL_THREADS, 0, s>>>(aa, grid);

@@ -889,8 +889,6 @@ __device__ __forceinline__ void render_fast_body(const Args &a) {
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     if (v.live) store_voice(a, dst, vi, v);
-    const uint64_t sounding = __ballot(v.live && v.stage != ENV_DEAD);
-    if (lane == 0) a.alive_rows[batch] = (uint32_t)__popcll(sounding);
 }
 // At most 128 VGPRs: four waves per SIMD, the occupancy the kernel is tuned for.
 template <int INTERP, bool CHAN, bool CLAMP>
