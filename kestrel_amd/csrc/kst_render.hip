@@ -469,58 +469,82 @@ __device__ __forceinline__ bool fade_in(const Voice &v, uint32_t f0) {
 // sub-tile. Every frame's phase wraps back by one loop span once its index
 // reaches the loop end, and a linear tap after the loop end is the loop start
 // (src/cpu.rs:243-259), as the edge path does it.
-template <int INTERP, bool CLAMP, int FILT, bool ENV_STATIC, bool FADE, bool WRAP = false>
-__device__ __forceinline__ void fast_subtile(const Args &a, Voice &v, float *tb,
-                                             const float *ftab, uint32_t lane, uint32_t f0) {
+// A lane with no voice, or a dead one, is neutralised for the straight path:
+// silent, and every tap it loads is the bank's first.
+template <int INTERP>
+__device__ __forceinline__ void neutralise(Voice &v) {
     if (!v.live || v.stage == ENV_DEAD) {
         v.gl = v.gr = 0.0f;
         v.step = 0;
         v.phase = (uint64_t)(INTERP == 2 ? 1u : 0u) << 32;
         v.base = 0;
     }
+}
+
+// Every frame's taps first: the phase chain is integer arithmetic, so all of
+// the sub-tile's loads are in flight before the first is needed. One copy of
+// this loop serves every straight-path variant (the dispatch loads the taps
+// and then picks the variant), so a sub-tile issues its loads once.
+template <int INTERP>
+__device__ __forceinline__ void straight_taps(const Args &a, const Voice &v,
+                                              uint32_t (&w0)[SUBTILE],
+                                              uint32_t (&w1)[INTERP == 2 ? SUBTILE : 1]) {
     const __amdgpu_buffer_rsrc_t bank = bank_rsrc(a);
-    // Every frame's taps first: the phase chain is integer arithmetic, so all
-    // of the sub-tile's loads can be in flight before the first is needed.
-    uint32_t w0[SUBTILE], w1[INTERP == 2 ? SUBTILE : 1];
-    uint64_t wrapped_end = 0;
-    if (WRAP) {
-        const bool sounding = v.live && v.stage != ENV_DEAD;
-        const bool looping = sounding && (v.vflags & VF_LOOP) != 0 &&
-                             !((v.vflags & VF_LOOP_UNTIL_RELEASE) != 0 && v.stage >= ENV_RELEASE);
-        const uint32_t le = looping ? v.le : 0xFFFFFFFFu;  // never reached when not looping
-        const uint32_t span = v.le - v.ls;
-        const uint32_t ls_tap = looping ? (uint32_t)(uint16_t)a.bank[v.base + v.ls] << 16 : 0u;
-        const uint32_t base2 = v.base * 2u;
-        uint64_t ph = v.phase;
+    // Byte offset 2 * (base + idx) as one 24-bit multiply-add of the phase's
+    // high word (idx < 2^24 on the straight path, which also keeps base + idx
+    // + reach in the bank). The phase chain is the one the frame loop runs, so
+    // the compiler shares it.
+    const uint32_t base2 = v.base * 2u - (INTERP == 2 ? 2u : 0u);
+    uint64_t ph = v.phase;
 #pragma unroll
-        for (uint32_t i = 0; i < SUBTILE; ++i) {
-            const uint32_t idx = (uint32_t)(ph >> 32);
-            uint32_t w = tap_pair(bank, __umul24(idx, a.tap_bytes) + base2);
-            if (INTERP == 1) w = idx + 1 == le ? (w & 0xFFFFu) | ls_tap : w;
-            w0[i] = w;
-            ph += v.step;
-            if ((uint32_t)(ph >> 32) >= le) ph -= (uint64_t)span << 32;
+    for (uint32_t i = 0; i < SUBTILE; ++i) {
+        const uint32_t off = __umul24((uint32_t)(ph >> 32), a.tap_bytes) + base2;
+        if (INTERP == 2) {
+            w0[i] = tap_pair(bank, off);      // frames idx-1, idx
+            w1[i] = tap_pair(bank, off + 4);  // frames idx+1, idx+2
+        } else {
+            w0[i] = tap_pair(bank, off);      // frames idx, idx+1
         }
-        wrapped_end = ph;
-    } else {
-        // Byte offset 2 * (base + idx) as one 24-bit multiply-add of the
-        // phase's high word (idx < 2^24 on the straight path, which also keeps
-        // base + idx + reach in the bank). The phase chain is the one the
-        // frame loop below runs, so the compiler shares it.
-        const uint32_t base2 = v.base * 2u - (INTERP == 2 ? 2u : 0u);
-        uint64_t ph = v.phase;
-#pragma unroll
-        for (uint32_t i = 0; i < SUBTILE; ++i) {
-            const uint32_t off = __umul24((uint32_t)(ph >> 32), a.tap_bytes) + base2;
-            if (INTERP == 2) {
-                w0[i] = tap_pair(bank, off);      // frames idx-1, idx
-                w1[i] = tap_pair(bank, off + 4);  // frames idx+1, idx+2
-            } else {
-                w0[i] = tap_pair(bank, off);      // frames idx, idx+1
-            }
-            ph += v.step;
-        }
+        ph += v.step;
     }
+}
+
+// The WRAP variant's taps (linear and nearest): each frame's phase wraps back
+// one loop span once its index reaches the loop end, and a linear tap after
+// the loop end is the loop start (src/cpu.rs:243-259). Returns the phase after
+// the sub-tile.
+template <int INTERP>
+__device__ __forceinline__ uint64_t wrap_taps(const Args &a, const Voice &v,
+                                              uint32_t (&w0)[SUBTILE]) {
+    const __amdgpu_buffer_rsrc_t bank = bank_rsrc(a);
+    const bool sounding = v.live && v.stage != ENV_DEAD;
+    const bool looping = sounding && (v.vflags & VF_LOOP) != 0 &&
+                         !((v.vflags & VF_LOOP_UNTIL_RELEASE) != 0 && v.stage >= ENV_RELEASE);
+    const uint32_t le = looping ? v.le : 0xFFFFFFFFu;  // never reached when not looping
+    const uint32_t span = v.le - v.ls;
+    const uint32_t ls_tap = looping ? (uint32_t)(uint16_t)a.bank[v.base + v.ls] << 16 : 0u;
+    const uint32_t base2 = v.base * 2u;
+    uint64_t ph = v.phase;
+#pragma unroll
+    for (uint32_t i = 0; i < SUBTILE; ++i) {
+        const uint32_t idx = (uint32_t)(ph >> 32);
+        uint32_t w = tap_pair(bank, __umul24(idx, a.tap_bytes) + base2);
+        if (INTERP == 1) w = idx + 1 == le ? (w & 0xFFFFu) | ls_tap : w;
+        w0[i] = w;
+        ph += v.step;
+        if ((uint32_t)(ph >> 32) >= le) ph -= (uint64_t)span << 32;
+    }
+    return ph;
+}
+
+// The straight path's frame loop over taps already loaded (neutralised lanes).
+// WRAP: the phase after the sub-tile is `wrapped_end` (see wrap_taps).
+template <int INTERP, bool CLAMP, int FILT, bool ENV_STATIC, bool FADE, bool WRAP = false>
+__device__ __forceinline__ void fast_subtile(const Args &a, Voice &v, float *tb,
+                                             const float *ftab, uint32_t lane, uint32_t f0,
+                                             const uint32_t (&w0)[SUBTILE],
+                                             const uint32_t (&w1)[INTERP == 2 ? SUBTILE : 1],
+                                             uint64_t wrapped_end = 0) {
     float m, c, thi, tlo;
     env_coefs(a, v, m, c, thi, tlo);
     const bool filt = FILT == 1 || (FILT == 2 && (v.p.flags & RP_FILTER));
@@ -618,15 +642,20 @@ __device__ __forceinline__ void fast_subtile_dispatch(const Args &a, Voice &v, f
     const bool sounding = v.live && v.stage != ENV_DEAD;
     const bool env_static = !FADE && !__any(sounding && v.stage != ENV_SUSTAIN);
     const bool f = (v.p.flags & RP_FILTER) != 0;
-    if (!__any(sounding && !f)) {
+    const bool all_filtered = !__any(sounding && !f);
+    const bool none_filtered = !__any(sounding && f);
+    neutralise<INTERP>(v);
+    uint32_t w0[SUBTILE], w1[INTERP == 2 ? SUBTILE : 1];
+    straight_taps<INTERP>(a, v, w0, w1);
+    if (all_filtered) {
         if (env_static)
-            fast_subtile<INTERP, CLAMP, 1, true, false>(a, v, tb, ftab, lane, f0);
+            fast_subtile<INTERP, CLAMP, 1, true, false>(a, v, tb, ftab, lane, f0, w0, w1);
         else
-            fast_subtile<INTERP, CLAMP, 1, false, FADE>(a, v, tb, ftab, lane, f0);
-    } else if (!__any(sounding && f)) {
-        fast_subtile<INTERP, CLAMP, 0, false, FADE>(a, v, tb, ftab, lane, f0);
+            fast_subtile<INTERP, CLAMP, 1, false, FADE>(a, v, tb, ftab, lane, f0, w0, w1);
+    } else if (none_filtered) {
+        fast_subtile<INTERP, CLAMP, 0, false, FADE>(a, v, tb, ftab, lane, f0, w0, w1);
     } else {
-        fast_subtile<INTERP, CLAMP, 2, false, FADE>(a, v, tb, ftab, lane, f0);
+        fast_subtile<INTERP, CLAMP, 2, false, FADE>(a, v, tb, ftab, lane, f0, w0, w1);
     }
 }
 
@@ -856,7 +885,10 @@ __device__ __forceinline__ void render_fast_body(const Args &a) {
         } else if (straight && a.fade_lds != 0) {
             fast_subtile_dispatch<INTERP, CLAMP, true>(a, v, tb, ftab, lane, f0);
         } else if (INTERP != 2 && !fading && !__any(!(fok || wrap_ok(a, v, f0)))) {
-            fast_subtile<INTERP, CLAMP, 2, false, false, true>(a, v, tb, ftab, lane, f0);
+            neutralise<INTERP>(v);
+            uint32_t w0[SUBTILE], w1[INTERP == 2 ? SUBTILE : 1];
+            const uint64_t end = wrap_taps<INTERP>(a, v, w0);
+            fast_subtile<INTERP, CLAMP, 2, false, false, true>(a, v, tb, ftab, lane, f0, w0, w1, end);
         } else if (INTERP != 2 && !__any(!edge_ok<INTERP>(a, v, f0))) {
             fast_subtile_edge<INTERP, CLAMP>(a, v, tb, ftab, lane, f0);
         } else {
