@@ -147,6 +147,22 @@ __device__ __forceinline__ bool sounding_at(const Args &a, const uint32_t *pool,
     return pool[F_ENV_STAGE * a.cap + i] != ENV_DEAD;
 }
 
+// Slot i's steal key and whether it is a candidate (sounding), with every load
+// issued unconditionally so a thread's slots are fetched together.
+__device__ __forceinline__ bool cand_at(const Args &a, const uint32_t *pool, uint32_t i,
+                                        uint32_t n, unsigned long long &key) {
+    if (i >= n) {
+        key = 0;
+        return false;
+    }
+    const bool sounding = pool[F_ENV_STAGE * a.cap + i] != ENV_DEAD;
+    key = steal_key(a, pool, i);
+    return sounding;
+}
+
+// Slots per thread fetched before any is used by the selection's passes.
+constexpr uint32_t SEL_UNROLL = 4;
+
 // ---------------------------------------------------------------------------
 // Steal selection: the steal_k smallest keys, which are unique (the note id is
 // in every key), so "key <= the k-th smallest" names exactly steal_k voices --
@@ -257,17 +273,20 @@ __device__ __forceinline__ void sel_hist_body(const Args &a) {
     const uint32_t *pool = sel2(a.pools, st->sel_parity);
     const uint32_t n = st->sel_n;
     const uint32_t lim = (n + 63) & ~63u;  // whole waves, so ballots see every lane
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < lim;
-         i += gridDim.x * blockDim.x) {
-        bool match = false;
-        unsigned long long key = 0;
-        uint32_t d = 0;
-        if (i < n && sounding_at(a, pool, i)) {
-            key = steal_key(a, pool, i);
-            match = sel_match(key, prefix, top);
-            d = (uint32_t)(key >> lo) & dmask;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x; i0 < lim; i0 += SEL_UNROLL * stride) {
+        unsigned long long key[SEL_UNROLL];
+        bool match[SEL_UNROLL];
+#pragma unroll
+        for (uint32_t u = 0; u < SEL_UNROLL; ++u)
+            match[u] = cand_at(a, pool, i0 + u * stride, n, key[u]);
+#pragma unroll
+        for (uint32_t u = 0; u < SEL_UNROLL; ++u) {
+            // Lanes of a wave agree on i < lim (lim and stride are whole waves).
+            if (i0 + u * stride >= lim) break;
+            const bool m = match[u] && sel_match(key[u], prefix, top);
+            sel_accumulate(m, key[u], (uint32_t)(key[u] >> lo) & dmask, cnt, mn, mx);
         }
-        sel_accumulate(match, key, d, cnt, mn, mx);
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < SEL_BINS; i += blockDim.x) {
@@ -420,26 +439,36 @@ __device__ __forceinline__ void sel_collect_body(const Args &a) {
     const unsigned long long pb = top >= 64 ? 0ull : prefix >> top;
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += gridDim.x * blockDim.x) {
-        if (!sounding_at(a, pool, i)) continue;
-        const unsigned long long kb = top >= 64 ? 0ull : steal_key(a, pool, i) >> top;
-        if (kb < pb) mark_victim(a, pool, i);
-        else if (kb == pb) atomicAdd(&s_n, 1u);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += SEL_UNROLL * stride) {
+        unsigned long long key[SEL_UNROLL];
+        bool ok[SEL_UNROLL];
+#pragma unroll
+        for (uint32_t u = 0; u < SEL_UNROLL; ++u) ok[u] = cand_at(a, pool, i0 + u * stride, n, key[u]);
+#pragma unroll
+        for (uint32_t u = 0; u < SEL_UNROLL; ++u) {
+            if (!ok[u]) continue;
+            const unsigned long long kb = top >= 64 ? 0ull : key[u] >> top;
+            if (kb < pb) mark_victim(a, pool, i0 + u * stride);
+            else if (kb == pb) atomicAdd(&s_n, 1u);
+        }
     }
     __syncthreads();
     if (threadIdx.x == 0) s_base = s_n ? atomicAdd(&st->cand_n, s_n) : 0;
     __syncthreads();
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += gridDim.x * blockDim.x) {
-        if (!sounding_at(a, pool, i)) continue;
-        const unsigned long long key = steal_key(a, pool, i);
-        if ((top >= 64 ? 0ull : key >> top) == pb) {
+    for (uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += SEL_UNROLL * stride) {
+        unsigned long long key[SEL_UNROLL];
+        bool ok[SEL_UNROLL];
+#pragma unroll
+        for (uint32_t u = 0; u < SEL_UNROLL; ++u) ok[u] = cand_at(a, pool, i0 + u * stride, n, key[u]);
+#pragma unroll
+        for (uint32_t u = 0; u < SEL_UNROLL; ++u) {
+            if (!ok[u] || (top >= 64 ? 0ull : key[u] >> top) != pb) continue;
             const uint32_t at = s_base + atomicAdd(&s_n, 1u);
-            a.cand_key[at] = key;
-            a.cand_slot[at] = i;
+            a.cand_key[at] = key[u];
+            a.cand_slot[at] = i0 + u * stride;
         }
     }
 }
