@@ -509,28 +509,43 @@ __device__ __forceinline__ void straight_taps(const Args &a, const Voice &v,
     }
 }
 
-// The WRAP variant's taps (linear and nearest): each frame's phase wraps back
-// one loop span once its index reaches the loop end, and a linear tap after
-// the loop end is the loop start (src/cpu.rs:243-259). Returns the phase after
+// The WRAP variant's taps: each frame's phase wraps back one loop span once
+// its index reaches the loop end, and a tap past the loop end reads from the
+// loop start (src/cpu.rs:243-259): for linear the tap after idx = le - 1 is
+// ls; for cubic idx + 1 and idx + 2 wrap the same way (ls, then ls + 1, or ls
+// again for a one-frame loop) while idx - 1 never does. Returns the phase after
 // the sub-tile.
 template <int INTERP>
 __device__ __forceinline__ uint64_t wrap_taps(const Args &a, const Voice &v,
-                                              uint32_t (&w0)[SUBTILE]) {
+                                              uint32_t (&w0)[SUBTILE],
+                                              uint32_t (&w1)[INTERP == 2 ? SUBTILE : 1]) {
     const __amdgpu_buffer_rsrc_t bank = bank_rsrc(a);
     const bool sounding = v.live && v.stage != ENV_DEAD;
     const bool looping = sounding && (v.vflags & VF_LOOP) != 0 &&
                          !((v.vflags & VF_LOOP_UNTIL_RELEASE) != 0 && v.stage >= ENV_RELEASE);
     const uint32_t le = looping ? v.le : 0xFFFFFFFFu;  // never reached when not looping
     const uint32_t span = v.le - v.ls;
-    const uint32_t ls_tap = looping ? (uint32_t)(uint16_t)a.bank[v.base + v.ls] << 16 : 0u;
     const uint32_t base2 = v.base * 2u;
+    // Frames ls and ls + 1 of a looping lane (ls twice for a one-frame loop).
+    uint32_t ls_pair = 0;
+    if (looping) {
+        const uint32_t p = INTERP == 2 ? tap_pair(bank, base2 + v.ls * 2u)
+                                       : (uint32_t)(uint16_t)a.bank[v.base + v.ls];
+        ls_pair = INTERP == 2 && span > 1 ? p : (p & 0xFFFFu) * 0x10001u;
+    }
     uint64_t ph = v.phase;
 #pragma unroll
     for (uint32_t i = 0; i < SUBTILE; ++i) {
         const uint32_t idx = (uint32_t)(ph >> 32);
-        uint32_t w = tap_pair(bank, __umul24(idx, a.tap_bytes) + base2);
-        if (INTERP == 1) w = idx + 1 == le ? (w & 0xFFFFu) | ls_tap : w;
-        w0[i] = w;
+        const uint32_t off = __umul24(idx, a.tap_bytes) + base2;
+        if (INTERP == 2) {
+            w0[i] = tap_pair(bank, off - 2);  // frames idx-1, idx (never past the loop end)
+            const uint32_t w = tap_pair(bank, off + 2);  // frames idx+1, idx+2
+            w1[i] = idx + 2 < le ? w : (idx + 2 == le ? (w & 0xFFFFu) | (ls_pair << 16) : ls_pair);
+        } else {
+            const uint32_t w = tap_pair(bank, off);
+            w0[i] = INTERP == 1 && idx + 1 == le ? (w & 0xFFFFu) | (ls_pair << 16) : w;
+        }
         ph += v.step;
         if ((uint32_t)(ph >> 32) >= le) ph -= (uint64_t)span << 32;
     }
@@ -695,6 +710,7 @@ __device__ __forceinline__ bool edge_ok(const Args &a, const Voice &v, uint32_t 
 // SUBTILE)? A started, looping lane whose loop is longer than its step (one
 // wrap per frame at most), with the tap offsets inside the 24-bit range and the
 // bank. Other lanes of the wave must pass fast_ok.
+template <int INTERP>
 __device__ __forceinline__ bool wrap_ok(const Args &a, const Voice &v, uint32_t f0) {
     if (!v.live || v.stage == ENV_DEAD) return true;
     if (v.start_rel > f0) return false;
@@ -703,6 +719,8 @@ __device__ __forceinline__ bool wrap_ok(const Args &a, const Voice &v, uint32_t 
     if (!looping) return false;
     const uint32_t hi = (uint32_t)(v.phase >> 32);
     if (v.ls >= v.le || hi >= v.le || (uint32_t)(v.step >> 32) >= v.le - v.ls) return false;
+    // Cubic's tap idx - 1 stays a plain read: the index never reaches 0.
+    if (INTERP == 2 && (hi == 0 || v.ls == 0)) return false;
     if (v.le >= (1u << 24)) return false;
     return (uint64_t)v.base + v.le < a.bank_len;
 }
@@ -884,10 +902,10 @@ __device__ __forceinline__ void render_fast_body(const Args &a) {
             fast_subtile_dispatch<INTERP, CLAMP, false>(a, v, tb, ftab, lane, f0);
         } else if (straight && a.fade_lds != 0) {
             fast_subtile_dispatch<INTERP, CLAMP, true>(a, v, tb, ftab, lane, f0);
-        } else if (INTERP != 2 && !fading && !__any(!(fok || wrap_ok(a, v, f0)))) {
+        } else if (!fading && !__any(!(fok || wrap_ok<INTERP>(a, v, f0)))) {
             neutralise<INTERP>(v);
             uint32_t w0[SUBTILE], w1[INTERP == 2 ? SUBTILE : 1];
-            const uint64_t end = wrap_taps<INTERP>(a, v, w0);
+            const uint64_t end = wrap_taps<INTERP>(a, v, w0, w1);
             fast_subtile<INTERP, CLAMP, 2, false, false, true>(a, v, tb, ftab, lane, f0, w0, w1, end);
         } else if (INTERP != 2 && !__any(!edge_ok<INTERP>(a, v, f0))) {
             fast_subtile_edge<INTERP, CLAMP>(a, v, tb, ftab, lane, f0);
