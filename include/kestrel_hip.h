@@ -407,16 +407,20 @@ int kst_driver_create(const kst_config *cfg, const kst_driver_config *dcfg,
                       const kst_midi_event *events, size_t n, kst_driver **out);
 /* Driver::next_block, src/driver.rs:434-642, against this library's backend:
  * returns 1 while the file or its tail has more, 0 when done, < 0 on failure.
- * `n` == block_frames * 2. While the device renders a block, the host already
- * drains the events of the next one and builds its rows (nothing of that
- * depends on the render); what each call returns and what kst_driver_get_stats
- * reports after it are those of the reference's sequential loop. */
+ * `n` == block_frames * 2. The driver's event side (events, rows, variants,
+ * candidates) runs one block ahead on a thread of its own, so the next block's
+ * events are drained while this one is admitted, rendered and limited (nothing
+ * of that depends on them); what each call returns and what
+ * kst_driver_get_stats reports after it are those of the reference's
+ * sequential loop. One caller thread at a time per driver. */
 int kst_driver_next_block(kst_driver *d, kst_backend *b, float *out, size_t n);
 /* The same block in the steps a caller with another backend drives itself:
  * begin (events, rows, variants; returns how many variants to upload), then
  * the rows and variants for that backend, admit (given its live voices and
  * stolen counter before spawn), spawn and render there, then end (limiter,
- * clamp, counters; returns 1 while there is more). */
+ * clamp, counters; returns 1 while there is more). The pointers the rows,
+ * pending-variant and admit calls hand out stay valid until the next begin;
+ * begin also starts the event side of the block after on the driver's thread. */
 int kst_driver_begin_block(kst_driver *d);
 int kst_driver_pending_variant(const kst_driver *d, size_t i, uint32_t *index,
                                const kst_region_params **data, size_t *n);
@@ -427,9 +431,10 @@ int kst_driver_admit(kst_driver *d, uint64_t live, uint64_t stolen_before,
 int kst_driver_end_block(kst_driver *d, float *out, size_t n, uint64_t stolen_after_spawn,
                          const kst_block_stats *after_render);
 int kst_driver_get_stats(const kst_driver *d, kst_driver_stats *out);
-/* Host time per block of kst_driver_next_block, by phase (events and rows,
- * admission and voice build, post-render step, the backend calls, the wait
- * for the device); writes up to `cap` entries and returns how many there are.
+/* Host time per block of kst_driver_next_block, by phase (events and rows on
+ * the event thread, the caller's wait for that thread, admission and voice
+ * build, post-render step, the backend calls, the wait for the device);
+ * writes up to `cap` entries and returns how many there are.
  * No counterpart in the reference. */
 size_t kst_driver_timings(const kst_driver *d, kst_timing *out, size_t cap);
 void kst_driver_destroy(kst_driver *d);

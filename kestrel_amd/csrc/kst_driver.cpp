@@ -9,13 +9,22 @@
 // what fits (evenly thinned or ranked by loudness within 64 time strata), build
 // only the admitted voices, and hand them to the backend. After the render:
 // the limiter, the hard clamp, and the end-of-file tail.
+//
+// The event side of a block (everything up to admission) reads nothing that
+// admission, the render or the post-render step of the block before it
+// writes, so it runs one block ahead on a thread of its own and hands each
+// block over in one swap of buffers.
 #include <algorithm>
 #include <array>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
+#include <exception>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 
 #include "kst_host.h"
 
@@ -314,30 +323,48 @@ public:
             if (kst_limiter_create(&lc, &limiter_) != KST_OK)
                 throw std::invalid_argument(kst_post_last_error());
         }
+        // The front starts as full-size rows: each swap hands the event side
+        // buffers of the size its tables write through.
+        f_.gate_rows = gates_.rows;
+        f_.chan_rows = chan_.rows;
+        events_ = std::thread([this] { event_thread(); });
     }
     ~Driver() {
+        {
+            std::unique_lock<std::mutex> g(m_);
+            cv_.wait(g, [&] { return !job_; });
+            stop_ = true;
+        }
+        cv_.notify_all();
+        events_.join();
         if (limiter_) kst_limiter_destroy(limiter_);
     }
 
     // Everything before admission (src/driver.rs:450-555). Returns how many
-    // params variants wait to be uploaded.
+    // params variants wait to be uploaded. The block was usually begun by the
+    // event thread while the caller finished the one before; its results are
+    // taken over here and that thread starts on the block after.
     size_t begin_block() {
-        if (ahead_) {  // already begun while the device rendered the last block
-            take_ahead_stats();
-            return uploads_.size();
-        }
-        return begin_at(block_index_);
+        if (ahead_)
+            join_ahead();  // rethrows what the event side threw
+        else
+            begin_at();
+        publish();
+        start_ahead();
+        return f_.uploads.size();
     }
 
-    size_t begin_at(uint64_t index) {
+    // The event side of the next block into the back buffers. Runs on the
+    // event thread, or on the caller's before the first block; it touches only
+    // state that admission, the render and the post-render step never do.
+    void begin_at() {
         const auto t0 = Clock::now();
         const uint64_t F = cfg_.block_frames;
-        const uint64_t block_start = index * F, block_end = block_start + F;
+        const uint64_t block_start = begin_index_ * F, block_end = block_start + F;
         gates_.begin_block();
         chan_.begin_block();
         variant_used_ = 1;
         for (uint32_t v : cur_variant_) variant_used_ |= 1ull << v;
-        spawn_buf_.clear();
         notes_.clear();
         cands_.clear();
         deferred_now_.clear();
@@ -378,7 +405,7 @@ public:
             }
             if (ev.kind == KST_EV_TEMPO) {
                 clock_.set_tempo(ev.tick, (uint32_t)ev.value);
-                ++stats_.events;
+                ++counts_.events;
                 continue;
             }
             const uint64_t frame = sat_u64(clock_.frame_at(ev.tick));
@@ -389,7 +416,7 @@ public:
             }
             const uint32_t rel = std::min<uint32_t>(
                 (uint32_t)(frame > block_start ? frame - block_start : 0), cfg_.block_frames - 1);
-            ++stats_.events;
+            ++counts_.events;
             handle_event(ev, rel, block_start);
         }
         // Past the end of the file everything is released, pedals or not.
@@ -406,15 +433,79 @@ public:
         chan_.end_block();
         apply_modulation();
         chan_.refresh_active();
-        t_events_ += Clock::now() - t0;
-        return uploads_.size();
+        ++begin_index_;
+        ev_time_ += Clock::now() - t0;
+    }
+
+    // Hands the back buffers over to admission (the front) in swaps; the
+    // event side overwrites every row of what it gets back before reading it.
+    void publish() {
+        f_.gate_rows.swap(gates_.rows);
+        f_.chan_rows.swap(chan_.rows);
+        f_.bend = chan_.bend_active;
+        f_.gain = chan_.gain_active;
+        f_.variant = chan_.variant_active || spawn_variants_;
+        f_.uploads.swap(uploads_);
+        f_.cands.swap(cands_);
+        f_.notes.swap(notes_);
+        f_.deferred_now.swap(deferred_now_);
+        f_.first_id = block_first_id_;
+        f_.src_more = !stream_done_ || !deferred_.empty();
+        stats_.events = counts_.events;
+        stats_.notes = counts_.notes;
+        stats_.variant_states = counts_.variant_states;
+        stats_.param_variants = counts_.param_variants;
+        stats_.variant_rebuilds = counts_.variant_rebuilds;
+        stats_.variant_fallbacks = counts_.variant_fallbacks;
+        t_events_ = ev_time_;
+    }
+
+    void start_ahead() {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            job_ = true;
+        }
+        cv_.notify_all();
+        ahead_ = true;
+    }
+
+    void join_ahead() {
+        const auto t0 = Clock::now();
+        std::unique_lock<std::mutex> g(m_);
+        cv_.wait(g, [&] { return !job_; });
+        ahead_ = false;
+        t_join_ += Clock::now() - t0;
+        if (err_) {
+            std::exception_ptr e = err_;
+            err_ = nullptr;
+            std::rethrow_exception(e);
+        }
+    }
+
+    void event_thread() {
+        std::unique_lock<std::mutex> g(m_);
+        for (;;) {
+            cv_.wait(g, [&] { return job_ || stop_; });
+            if (stop_) return;
+            g.unlock();
+            std::exception_ptr e;
+            try {
+                begin_at();
+            } catch (...) {
+                e = std::current_exception();
+            }
+            g.lock();
+            err_ = e;
+            job_ = false;
+            cv_.notify_all();
+        }
     }
 
     // Admission and the build of what it admits (src/driver.rs:556-594).
     void admit(uint64_t live64, uint64_t stolen_before) {
         const auto t0 = Clock::now();
         const uint32_t live = (uint32_t)live64;
-        const size_t want = cands_.size();
+        const size_t want = f_.cands.size();
         const size_t take = admit_take(cfg_, live, (uint32_t)want);
         stats_.dropped += want - take;
         stolen_before_ = stolen_before;
@@ -423,25 +514,28 @@ public:
         stats_.last_take = (uint32_t)take;
         if (take < want && take > 0) {
             if (cfg_.admit_rule == 1) {  // Even: thin by position
-                for (size_t i = 0; i < take; ++i) cands_[i] = cands_[(size_t)((uint64_t)i * want / take)];
+                for (size_t i = 0; i < take; ++i) f_.cands[i] = f_.cands[(size_t)((uint64_t)i * want / take)];
             } else {
                 rank_candidates(want, take);
             }
         }
-        stats_.last_want_energy = stats_.last_take_energy = 0;
-        for (size_t i = 0; i < cands_.size(); ++i) {
-            const uint64_t e = (cands_[i].key >> 48) & 0x7FFF;
-            stats_.last_want_energy += e;
-            if (i < take) stats_.last_take_energy += e;
+        uint64_t want_e = 0, take_e = 0;
+        for (size_t i = 0; i < f_.cands.size(); ++i) {
+            const uint64_t e = (f_.cands[i].key >> 48) & 0x7FFF;
+            want_e += e;
+            if (i < take) take_e += e;
         }
-        materialise(std::min(take, cands_.size()));
+        stats_.last_want_energy = want_e;
+        stats_.last_take_energy = take_e;
+        spawn_buf_.clear();
+        materialise(std::min(take, f_.cands.size()));
         stats_.voices_spawned += want;
         t_admit_ += Clock::now() - t0;
     }
 
     // After the render (src/driver.rs:596-641). Returns whether there is more.
     int end_block(float *out, size_t n, uint64_t stolen_after_spawn, const kst_block_stats &st) {
-        return end_at(out, n, stolen_after_spawn, st, !stream_done_ || !deferred_.empty());
+        return end_at(out, n, stolen_after_spawn, st, f_.src_more);
     }
 
     // `src_more`: the file or the deferred queue still had something after
@@ -476,25 +570,31 @@ public:
         return 0;
     }
 
-    // One block against this library's backend. The device renders block b
-    // while the host drains the events of b+1 and builds its rows: none of
-    // that reads the render, and its counters are held back so that what the
-    // caller sees after each call is what the sequential loop would show.
+    // One block against this library's backend. The event thread drains the
+    // events of block b+1 and builds its rows while this thread admits block
+    // b, uploads it, waits for the device and runs the post-render step: none
+    // of that reads what the other side writes, and the counters of b+1 are
+    // held on the event side until begin_block takes the block over, so what
+    // each call returns and reports is what the sequential loop would show.
     int next_block(kst_backend *b, float *out, size_t n) {
         if (n != (size_t)cfg_.block_frames * 2)
             return fail(KST_EINVAL, "output block is " + std::to_string(n) + " samples, expected " +
                                         std::to_string(cfg_.block_frames * 2));
-        begin_block();
+        try {
+            begin_block();
+        } catch (const std::exception &e) {
+            return fail(KST_EINVAL, e.what());
+        }
         auto be = [](int rc) {
             if (rc != KST_OK) throw std::runtime_error(kst_last_error());
         };
         try {
             auto tb = Clock::now();
-            for (const auto &u : uploads_)
+            for (const auto &u : f_.uploads)
                 be(kst_set_params_variant(b, u.first, u.second.data(), u.second.size()));
-            be(kst_set_gates(b, gates_.rows.data(), gates_.rows.size()));
-            be(kst_set_channels(b, chan_.rows.data(), chan_.rows.size(), chan_.bend_active,
-                                chan_.gain_active, chan_.variant_active || spawn_variants_));
+            be(kst_set_gates(b, f_.gate_rows.data(), f_.gate_rows.size()));
+            be(kst_set_channels(b, f_.chan_rows.data(), f_.chan_rows.size(), f_.bend, f_.gain,
+                                f_.variant));
             kst_block_stats st{};
             be(kst_stats(b, &st));
             t_backend_ += Clock::now() - tb;
@@ -505,18 +605,12 @@ public:
             const uint64_t stolen_after = st.stolen;
             be(kst_render_submit(b));
             t_backend_ += Clock::now() - tb;
-            const bool src_more = !stream_done_ || !deferred_.empty();
-            const kst_driver_stats published = stats_;
-            begin_at(block_index_ + 1);
-            ahead_stats_ = stats_;
-            stats_ = published;
-            ahead_ = true;
             tb = Clock::now();
             be(kst_render_collect(b, out, n));
             be(kst_stats(b, &st));
             t_wait_ += Clock::now() - tb;
             ++timed_blocks_;
-            return end_at(out, n, stolen_after, st, src_more);
+            return end_at(out, n, stolen_after, st, f_.src_more);
         } catch (const std::runtime_error &e) {
             return fail(KST_EDEVICE, e.what());
         }
@@ -525,22 +619,28 @@ public:
     size_t timings(kst_timing *out, size_t cap) const {
         if (timed_blocks_ == 0) return 0;
         const double k = 1e-6 / (double)timed_blocks_;
-        const kst_timing t[5] = {
+        const kst_timing t[6] = {
             {"events_rows", std::chrono::duration<double, std::nano>(t_events_).count() * k},
+            {"events_wait", std::chrono::duration<double, std::nano>(t_join_).count() * k},
             {"admission_build", std::chrono::duration<double, std::nano>(t_admit_).count() * k},
             {"post_render", std::chrono::duration<double, std::nano>(t_post_).count() * k},
             {"backend_calls", std::chrono::duration<double, std::nano>(t_backend_).count() * k},
             {"device_wait", std::chrono::duration<double, std::nano>(t_wait_).count() * k}};
-        for (size_t i = 0; i < 5 && i < cap; ++i) out[i] = t[i];
-        return 5;
+        for (size_t i = 0; i < 6 && i < cap; ++i) out[i] = t[i];
+        return 6;
     }
 
+    // The block begin_block took over: valid until the next begin_block.
     const std::vector<std::pair<uint32_t, std::vector<kst_region_params>>> &uploads() const {
-        return uploads_;
+        return f_.uploads;
     }
-    const GateTable &gates() const { return gates_; }
-    const ChannelTable &chan() const { return chan_; }
-    bool spawn_variants() const { return spawn_variants_; }
+    const std::vector<uint32_t> &gate_rows() const { return f_.gate_rows; }
+    const std::vector<uint32_t> &chan_rows() const { return f_.chan_rows; }
+    void chan_flags(uint32_t flags[3]) const {
+        flags[0] = f_.bend;
+        flags[1] = f_.gain;
+        flags[2] = f_.variant;
+    }
     const std::vector<kst_spawn_cmd> &spawned() const { return spawn_buf_; }
     const kst_driver_stats &stats() const { return stats_; }
     uint32_t block_frames() const { return cfg_.block_frames; }
@@ -559,15 +659,17 @@ private:
     };
     typedef std::array<uint8_t, 5> State;
 
+    // Read-only after construction.
     kst_config cfg_;
     kst_driver_config dcfg_;
     const Bank &bank_;
-    std::unique_ptr<EventSource> src_;
+
+    // Event side: the event thread's own state (or the caller's before the
+    // first block), kept off the cache lines the caller's side writes.
+    alignas(128) std::unique_ptr<EventSource> src_;
     TempoClock clock_;
     GateTable gates_;
     ChannelTable chan_;
-    kst_limiter *limiter_ = nullptr;
-
     uint8_t bank_msb_[16] = {}, bank_lsb_[16] = {};
     uint32_t preset_[16] = {};
     int16_t bend_val_[16] = {};
@@ -584,39 +686,52 @@ private:
     std::vector<uint64_t> variant_seen_;
     uint64_t variant_clock_ = 0;
     std::vector<std::pair<uint32_t, ParamMod>> pending_variants_;
-    std::vector<std::pair<uint32_t, std::vector<kst_region_params>>> uploads_;
-    bool spawn_variants_ = false;
-
-    uint64_t next_note_id_ = 1, block_index_ = 0;
+    uint64_t next_note_id_ = 1;
     kst_midi_event pending_{};
     bool have_pending_ = false, stream_done_ = false, end_sent_ = false;
-    uint64_t tail_blocks_left_ = 0, stolen_before_ = 0;
-
-    std::vector<kst_spawn_cmd> spawn_buf_, deferred_now_;
+    std::vector<Preview> preview_;
+    std::vector<std::pair<uint64_t, kst_spawn_cmd>> deferred_;
+    uint64_t begin_index_ = 0;  // the block begin_at drains next
+    // The back buffers begin_at fills, handed over by publish().
+    std::vector<std::pair<uint32_t, std::vector<kst_region_params>>> uploads_;
+    bool spawn_variants_ = false;
+    std::vector<kst_spawn_cmd> deferred_now_;
     std::vector<Note> notes_;
     std::vector<Cand> cands_;
-    std::vector<Preview> preview_;
     uint64_t block_first_id_ = 0;
-    std::vector<std::pair<uint64_t, kst_spawn_cmd>> deferred_;
-    kst_driver_stats stats_{};
-    // The next block, begun early by next_block, and its counters.
-    bool ahead_ = false;
-    kst_driver_stats ahead_stats_{};
+    // Event-side counters (events, notes, variant bookkeeping), copied into
+    // stats_ as each block is taken over.
+    kst_driver_stats counts_{};
     typedef std::chrono::steady_clock Clock;
-    Clock::duration t_events_{}, t_admit_{}, t_post_{}, t_backend_{}, t_wait_{};
+    Clock::duration ev_time_{};  // event side's own total, copied on take-over
+
+    // The caller's side: admission, the backend calls and the post-render step.
+    // One block's event-side results as admission and the backend read them.
+    struct Front {
+        std::vector<uint32_t> gate_rows, chan_rows;
+        bool bend = false, gain = false, variant = false;
+        std::vector<std::pair<uint32_t, std::vector<kst_region_params>>> uploads;
+        std::vector<Cand> cands;
+        std::vector<Note> notes;
+        std::vector<kst_spawn_cmd> deferred_now;
+        uint64_t first_id = 0;
+        bool src_more = true;  // the file or the deferred queue had more after it
+    };
+    alignas(128) Front f_;
+    std::vector<kst_spawn_cmd> spawn_buf_;
+    kst_driver_stats stats_{};
+    kst_limiter *limiter_ = nullptr;
+    uint64_t block_index_ = 0, tail_blocks_left_ = 0, stolen_before_ = 0;
+    Clock::duration t_events_{}, t_admit_{}, t_post_{}, t_backend_{}, t_wait_{}, t_join_{};
     uint64_t timed_blocks_ = 0;
 
-    // Counters begin_at moved for the early block: events, notes and the
-    // variant bookkeeping. Nothing after it in a block touches those.
-    void take_ahead_stats() {
-        stats_.events = ahead_stats_.events;
-        stats_.notes = ahead_stats_.notes;
-        stats_.variant_states = ahead_stats_.variant_states;
-        stats_.param_variants = ahead_stats_.param_variants;
-        stats_.variant_rebuilds = ahead_stats_.variant_rebuilds;
-        stats_.variant_fallbacks = ahead_stats_.variant_fallbacks;
-        ahead_ = false;
-    }
+    // The event thread and its one-job handshake: job_ is set while it
+    // begins a block, err_ holds what that threw.
+    alignas(128) std::mutex m_;
+    std::condition_variable cv_;
+    bool job_ = false, stop_ = false, ahead_ = false;
+    std::exception_ptr err_;
+    std::thread events_;
 
     static State neutral_state() {
         State s;
@@ -672,34 +787,34 @@ private:
             if (q == 0) continue;
             if (q < n) {
                 keys.resize(n);
-                for (size_t i = 0; i < n; ++i) keys[i] = cands_[lo + i].key;
+                for (size_t i = 0; i < n; ++i) keys[i] = f_.cands[lo + i].key;
                 std::nth_element(keys.begin(), keys.begin() + (q - 1), keys.end(),
                                  std::greater<uint64_t>());
                 const uint64_t kth = keys[q - 1];  // keys are distinct: exactly q at or above
                 tmp.clear();
                 for (size_t i = lo; i < hi; ++i)
-                    if (cands_[i].key >= kth) tmp.push_back(cands_[i]);
+                    if (f_.cands[i].key >= kth) tmp.push_back(f_.cands[i]);
                 for (size_t i = lo; i < hi; ++i)
-                    if (cands_[i].key < kth) tmp.push_back(cands_[i]);
-                std::copy(tmp.begin(), tmp.end(), cands_.begin() + lo);
+                    if (f_.cands[i].key < kth) tmp.push_back(f_.cands[i]);
+                std::copy(tmp.begin(), tmp.end(), f_.cands.begin() + lo);
             }
-            for (size_t j = 0; j < q; ++j) std::swap(cands_[olo + j], cands_[lo + j]);
+            for (size_t j = 0; j < q; ++j) std::swap(f_.cands[olo + j], f_.cands[lo + j]);
         }
     }
 
     void materialise(size_t take) {  // src/driver.rs:751-783
         for (size_t i = 0; i < take; ++i) {
-            const Cand &c = cands_[i];
+            const Cand &c = f_.cands[i];
             if (c.note == DEFERRED) {
-                spawn_buf_.push_back(deferred_now_[c.region]);
+                spawn_buf_.push_back(f_.deferred_now[c.region]);
                 continue;
             }
-            const Note &nt = notes_[c.note];
+            const Note &nt = f_.notes[c.note];
             Layer v;
             if (!bank_.build_layer(c.region, nt.key, nt.vel, v)) continue;
             const uint32_t start_rel = v.delay_frames == 0 ? nt.rel : nt.rel + v.delay_frames;
             spawn_buf_.push_back(make_cmd(v, nt.variant, GateTable::slot(nt.ch, nt.key), nt.ordinal,
-                                          start_rel, block_first_id_ + c.id_off));
+                                          start_rel, f_.first_id + c.id_off));
         }
     }
 
@@ -710,7 +825,7 @@ private:
         case KST_EV_NOTE_ON: {  // src/driver.rs:787-872
             const uint8_t key = ev.a, vel = ev.b;
             const uint32_t ordinal = gates_.note_on(ch, key, rel);
-            ++stats_.notes;
+            ++counts_.notes;
             const uint32_t variant = cur_variant_[c];
             preview_.clear();
             bank_.preview_note_on(preset_[c], key, vel, dcfg_.max_layers, preview_);
@@ -816,7 +931,7 @@ private:
         for (int i = 0; i < 5; ++i) want[i] = (uint8_t)(cc_sound_[c][i] >> SOUND_CC_SHIFT);
         if (std::find(seen_states_.begin(), seen_states_.end(), want) == seen_states_.end()) {
             seen_states_.push_back(want);
-            stats_.variant_states = (uint32_t)seen_states_.size();
+            counts_.variant_states = (uint32_t)seen_states_.size();
         }
         const uint32_t cap = std::min<uint32_t>(std::max<uint32_t>(cfg_.max_param_variants, 1), 63);
         uint32_t idx;
@@ -827,7 +942,7 @@ private:
             idx = (uint32_t)variants_.size();
             variants_.push_back(want);
             variant_seen_.push_back(0);
-            stats_.param_variants = idx + 1;
+            counts_.param_variants = idx + 1;
             build_variant_at(idx, c);
         } else {
             // Reuse the stalest slot no row of this block names yet.
@@ -840,9 +955,9 @@ private:
                 idx = (uint32_t)victim;
                 variants_[idx] = want;
                 build_variant_at(idx, c);
-                ++stats_.variant_rebuilds;
+                ++counts_.variant_rebuilds;
             } else {  // every slot is spoken for: the nearest state
-                ++stats_.variant_fallbacks;
+                ++counts_.variant_fallbacks;
                 uint32_t best = 0, best_d = UINT32_MAX;
                 for (size_t i = 0; i < variants_.size(); ++i) {
                     uint32_t d = 0;
@@ -1184,7 +1299,11 @@ int kst_driver_next_block(kst_driver *d, kst_backend *b, float *out, size_t n) {
 
 int kst_driver_begin_block(kst_driver *d) {
     if (!d) return fail(KST_EINVAL, "null driver");
-    return (int)d->d->begin_block();
+    try {
+        return (int)d->d->begin_block();
+    } catch (const std::exception &e) {
+        return fail(KST_EINVAL, e.what());
+    }
 }
 
 int kst_driver_pending_variant(const kst_driver *d, size_t i, uint32_t *index,
@@ -1201,13 +1320,11 @@ int kst_driver_pending_variant(const kst_driver *d, size_t i, uint32_t *index,
 int kst_driver_rows(const kst_driver *d, const uint32_t **gates, size_t *n_gates,
                     const uint32_t **chans, size_t *n_chans, uint32_t flags[3]) {
     if (!d || !gates || !n_gates || !chans || !n_chans || !flags) return fail(KST_EINVAL, "null argument");
-    *gates = d->d->gates().rows.data();
-    *n_gates = d->d->gates().rows.size();
-    *chans = d->d->chan().rows.data();
-    *n_chans = d->d->chan().rows.size();
-    flags[0] = d->d->chan().bend_active;
-    flags[1] = d->d->chan().gain_active;
-    flags[2] = d->d->chan().variant_active || d->d->spawn_variants();
+    *gates = d->d->gate_rows().data();
+    *n_gates = d->d->gate_rows().size();
+    *chans = d->d->chan_rows().data();
+    *n_chans = d->d->chan_rows().size();
+    d->d->chan_flags(flags);
     return KST_OK;
 }
 

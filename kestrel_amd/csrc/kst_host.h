@@ -211,7 +211,9 @@ Bank *load_sfz(const BankSettings &set, const std::string &path, bool resample_p
 
 // Decodes another source ahead on a thread of its own, in chunks through a
 // bounded queue: the consumer sees exactly the inner sequence, and the file's
-// decoding leaves the block loop's critical path.
+// decoding leaves the block loop's critical path. Consumed chunks go back to
+// the decoder for reuse, so a long file streams through a fixed set of
+// buffers (no allocation, no fresh pages) once the queue has filled.
 class Prefetch : public EventSource {
 public:
     explicit Prefetch(std::unique_ptr<EventSource> inner, size_t chunk = 1 << 15, size_t depth = 32);
@@ -225,6 +227,7 @@ private:
     std::mutex m_;
     std::condition_variable cv_;
     std::deque<std::vector<kst_midi_event>> q_;
+    std::vector<std::vector<kst_midi_event>> spare_;  // consumed chunks to refill
     bool done_ = false, stop_ = false;
     std::vector<kst_midi_event> cur_;
     size_t at_ = 0;

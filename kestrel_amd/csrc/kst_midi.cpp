@@ -346,6 +346,14 @@ Prefetch::~Prefetch() {
 void Prefetch::run() {
     for (;;) {
         std::vector<kst_midi_event> c;
+        {
+            std::lock_guard<std::mutex> g(m_);
+            if (!spare_.empty()) {
+                c = std::move(spare_.back());
+                spare_.pop_back();
+            }
+        }
+        c.clear();
         c.reserve(chunk_);
         kst_midi_event e;
         while (c.size() < chunk_ && inner_->next(e)) c.push_back(e);
@@ -369,6 +377,7 @@ bool Prefetch::next(kst_midi_event &ev) {
     std::unique_lock<std::mutex> g(m_);
     cv_.wait(g, [&] { return !q_.empty() || done_; });
     if (q_.empty()) return false;
+    if (cur_.capacity()) spare_.push_back(std::move(cur_));
     cur_ = std::move(q_.front());
     q_.pop_front();
     g.unlock();

@@ -331,6 +331,41 @@ def test_driver_from_a_file_matches_the_oracle(tmp_path):
     assert st["notes"] == 400 and st["events"] == len(oev)
 
 
+def test_driver_block_stays_put_while_the_next_one_begins():
+    """begin_block starts the next block's event side on the driver's thread:
+    what the caller holds of the current block (rows, flags, variants, spawn
+    list) must not move while that runs, and pacing the calls differently
+    must not change a single output or counter."""
+    import time
+    from kestrel_amd.driver import Division, Driver
+    cfg = DC.driver_config(max_voices=60, max_param_variants=3)
+    desc = DC.make_bank_desc(55)
+    ev = DC.random_events(56, 1500, 1200, cc_rate=0.3)
+    lb = lib_bank(cfg, desc)
+    runs = []
+    for pause in (0.0, 0.003):
+        ld = Driver(cfg, lb, division=Division.ppq_of(480), events=ev)
+        seen = []
+        for b in range(120):
+            live = cfg.max_voices - 10 if b % 2 else 0  # every other block admission chooses
+            ups = ld.begin_block()
+            g1, c1, f1 = ld.rows()
+            cmds = ld.admit(live, 0)
+            if pause:
+                time.sleep(pause)  # the event thread finishes the next block meanwhile
+            g2, c2, f2 = ld.rows()
+            assert np.array_equal(g1, g2) and np.array_equal(c1, c2) and f1 == f2, b
+            out = np.full(cfg.block_frames * 2, 0.25, np.float32)
+            more = ld.end_block(out, 0, dict(active_voices=live, stolen=0, dropped=0, peak=0.25))
+            seen.append((g1.tobytes(), c1.tobytes(), f1, [(i, t.tobytes()) for i, t in ups],
+                         cmds.tobytes(), more))
+        runs.append((seen, ld.stats()))
+        ld.close()
+    assert runs[0][0] == runs[1][0]
+    assert runs[0][1] == runs[1][1]
+    assert runs[0][1]["dropped"] > 0 and runs[0][1]["notes"] > 100
+
+
 def test_driver_validates_its_inputs():
     from kestrel_amd.abi import BackendError
     from kestrel_amd.driver import Division, Driver
