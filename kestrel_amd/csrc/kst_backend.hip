@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "kst_device.h"
+#include "kst_stage.h"
 
 using namespace kst;
 
@@ -197,7 +198,7 @@ struct kst_backend {
     DevBuf<BlockResult> g_results;
 
     // Host side of the per-block API.
-    std::vector<uint32_t> h_gates, h_chans;
+    uint32_t *h_gates = nullptr, *h_chans = nullptr;  // pinned: uploads run without staging
     uint32_t h_chan_flags = 0;
     bool staged_chan = true, graph_chan = true;
     size_t n_pending = 0;  // this block's spawn commands, staged in pin_cmds
@@ -263,6 +264,8 @@ struct kst_backend {
         g_out.release();
         g_results.release();
         if (pin_small) (void)hipHostFree(pin_small);
+        if (h_gates) (void)hipHostFree(h_gates);
+        if (h_chans) (void)hipHostFree(h_chans);
         if (pin_cmds) (void)hipHostFree(pin_cmds);
         if (pin_out) (void)hipHostFree(pin_out);
         if (pin_result) (void)hipHostFree(pin_result);
@@ -475,6 +478,46 @@ struct kst_backend {
     }
 };
 
+namespace kst {
+
+// Room for n commands in the pinned staging the render uploads from.
+int spawn_reserve(kst_backend *b, size_t n, kst_spawn_cmd **out) {
+    if (!b || !out) return fail(KST_EINVAL, "null argument");
+    if (b->inflight) return fail(KST_ESTATE, "a submitted render has not been collected");
+    if (n > 0xFFFFFFFFull) return fail(KST_EINVAL, "%zu spawn commands in one block", n);
+    if (n > b->pin_cmds_cap || !b->pin_cmds) {
+        if (b->pin_cmds) (void)hipHostFree(b->pin_cmds);
+        b->pin_cmds = nullptr;
+        b->pin_cmds_cap = 0;
+        const size_t cap = std::max<size_t>(n + n / 4, 1024);
+        HIPCHK(hipHostMalloc(&b->pin_cmds, cap * sizeof(kst_spawn_cmd)));
+        b->pin_cmds_cap = cap;
+    }
+    *out = b->pin_cmds;
+    return KST_OK;
+}
+
+// The first n staged commands are this block's; the eager accounting of
+// src/cpu.rs:330-390, which the device repeats in k_begin from the same inputs.
+int spawn_commit(kst_backend *b, size_t n) {
+    if (!b) return fail(KST_EINVAL, "null argument");
+    if (b->inflight) return fail(KST_ESTATE, "a submitted render has not been collected");
+    if (n > b->pin_cmds_cap) return fail(KST_EINVAL, "%zu spawn commands past the %zu reserved", n,
+                                         b->pin_cmds_cap);
+    b->n_pending = n;
+    const uint64_t cap = b->cfg.max_voices, live = b->live, raw = n;
+    const uint64_t want = std::min<uint64_t>(raw, cap);
+    uint64_t need = 0;
+    if (live + want > cap && b->cfg.steal_rule != STEAL_DROP_NEW)
+        need = std::min<uint64_t>(std::min<uint64_t>(live + want - cap, live), b->max_steal);
+    const uint64_t take = std::min<uint64_t>(want, cap - (live - need));
+    b->stolen += need;
+    b->dropped += raw - take;
+    return KST_OK;
+}
+
+}  // namespace kst
+
 extern "C" {
 
 void kst_default_config(kst_config *c) {
@@ -642,12 +685,14 @@ int kst_create(const kst_config *cfg, const int16_t *pool, size_t pool_len,
         return cleanup(fail(KST_EDEVICE, "initialising the device state"));
     if (b->ensure_stage(1, 1024) != KST_OK) return cleanup(KST_EDEVICE);
 
-    b->h_gates.assign(b->gate_words, 0);
-    b->h_chans.assign(b->chan_words, 0);
-    if (hipHostMalloc(&b->pin_small, 4096 * sizeof(uint32_t)) != hipSuccess ||
+    if (hipHostMalloc(&b->h_gates, (size_t)b->gate_words * 4) != hipSuccess ||
+        hipHostMalloc(&b->h_chans, (size_t)b->chan_words * 4) != hipSuccess ||
+        hipHostMalloc(&b->pin_small, 4096 * sizeof(uint32_t)) != hipSuccess ||
         hipHostMalloc(&b->pin_out, two_f * sizeof(float)) != hipSuccess ||
         hipHostMalloc(&b->pin_result, sizeof(BlockResult)) != hipSuccess)
         return cleanup(fail(KST_EDEVICE, "allocating pinned staging"));
+    std::memset(b->h_gates, 0, (size_t)b->gate_words * 4);
+    std::memset(b->h_chans, 0, (size_t)b->chan_words * 4);
     for (auto &x : b->ev)
         if (hipEventCreate(&x) != hipSuccess)
             return cleanup(fail(KST_EDEVICE, "creating timing events"));
@@ -668,7 +713,7 @@ int kst_set_gates(kst_backend *b, const uint32_t *rows, size_t n) {
     if (!b || (!rows && n)) return fail(KST_EINVAL, "null argument");
     if (n != b->gate_words)
         return fail(KST_EINVAL, "gate rows are %zu words, expected %u", n, b->gate_words);
-    std::memcpy(b->h_gates.data(), rows, n * 4);
+    std::memcpy(b->h_gates, rows, n * 4);
     return KST_OK;
 }
 
@@ -678,7 +723,7 @@ int kst_set_channels(kst_backend *b, const uint32_t *rows, size_t n, int bend, i
     if (!b || (!rows && n)) return fail(KST_EINVAL, "null argument");
     if (n != b->chan_words)
         return fail(KST_EINVAL, "channel rows are %zu words, expected %u", n, b->chan_words);
-    std::memcpy(b->h_chans.data(), rows, n * 4);
+    std::memcpy(b->h_chans, rows, n * 4);
     b->h_chan_flags = (bend ? CHAN_ACTIVE_BEND : 0) | (gain ? CHAN_ACTIVE_GAIN : 0) |
                       (variant ? CHAN_ACTIVE_VARIANT : 0);
     return KST_OK;
@@ -738,29 +783,10 @@ int kst_params_export(kst_backend *b, uint32_t index, kst_region_params *out, si
 int kst_spawn(kst_backend *b, const kst_spawn_cmd *cmds, size_t n) {
     if (b && b->inflight) return fail(KST_ESTATE, "a submitted render has not been collected");
     if (!b || (!cmds && n)) return fail(KST_EINVAL, "null argument");
-    if (n > 0xFFFFFFFFull) return fail(KST_EINVAL, "%zu spawn commands in one block", n);
-    // Straight into the pinned staging buffer the render uploads from.
-    if (n > b->pin_cmds_cap) {
-        if (b->pin_cmds) (void)hipHostFree(b->pin_cmds);
-        b->pin_cmds = nullptr;
-        b->pin_cmds_cap = 0;
-        const size_t cap = std::max<size_t>(n + n / 4, 1024);
-        HIPCHK(hipHostMalloc(&b->pin_cmds, cap * sizeof(kst_spawn_cmd)));
-        b->pin_cmds_cap = cap;
-    }
-    if (n) std::memcpy(b->pin_cmds, cmds, n * sizeof(kst_spawn_cmd));
-    b->n_pending = n;
-    // Eager accounting, src/cpu.rs:330-390; the device repeats this arithmetic
-    // in k_begin from the same inputs.
-    const uint64_t cap = b->cfg.max_voices, live = b->live, raw = n;
-    const uint64_t want = std::min<uint64_t>(raw, cap);
-    uint64_t need = 0;
-    if (live + want > cap && b->cfg.steal_rule != STEAL_DROP_NEW)
-        need = std::min<uint64_t>(std::min<uint64_t>(live + want - cap, live), b->max_steal);
-    const uint64_t take = std::min<uint64_t>(want, cap - (live - need));
-    b->stolen += need;
-    b->dropped += raw - take;
-    return KST_OK;
+    kst_spawn_cmd *dst = nullptr;
+    if (int rc = kst::spawn_reserve(b, n, &dst)) return rc;
+    if (n) std::memcpy(dst, cmds, n * sizeof(kst_spawn_cmd));
+    return kst::spawn_commit(b, n);
 }
 
 int kst_render_submit(kst_backend *b) {
@@ -773,21 +799,16 @@ int kst_render_submit(kst_backend *b) {
     if (rc) return rc;
     hipStream_t s = b->stream;
     // Block slot 0: gates, channels, flags, commands, and the block index.
-    HIPCHK(hipMemcpyAsync(b->g_gates.p, b->h_gates.data(), b->gate_words * 4,
+    HIPCHK(hipMemcpyAsync(b->g_gates.p, b->h_gates, b->gate_words * 4,
                           hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(b->g_chans.p, b->h_chans.data(), b->chan_words * 4,
+    HIPCHK(hipMemcpyAsync(b->g_chans.p, b->h_chans, b->chan_words * 4,
                           hipMemcpyHostToDevice, s));
     if (ncmd)
         HIPCHK(hipMemcpyAsync(b->g_cmds.p, b->pin_cmds, (size_t)ncmd * sizeof(kst_spawn_cmd),
                               hipMemcpyHostToDevice, s));
-    b->pin_small[0] = b->h_chan_flags;
-    b->pin_small[1] = ncmd;
-    b->pin_small[2] = 0;  // command offset
-    b->pin_small[3] = 0;  // block index
-    HIPCHK(hipMemcpyAsync(b->g_chan_flags.p, &b->pin_small[0], 4, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(b->g_cmd_counts.p, &b->pin_small[1], 4, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(b->g_cmd_offs.p, &b->pin_small[2], 4, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(&b->st.p->block, &b->pin_small[3], 4, hipMemcpyHostToDevice, s));
+    // Flags, command count, command offset 0 and block index 0, as kernel
+    // arguments rather than four copies.
+    launch_stage_one(b->args(1), b->h_chan_flags, ncmd, s);
 
     // Host copy of k_begin's arithmetic, for grid sizes only.
     const uint64_t cap = b->cfg.max_voices, live = b->live;

@@ -211,6 +211,9 @@ void launch_select(const Args &a, uint32_t grid, uint32_t rounds, hipStream_t s)
 void launch_spawn(const Args &a, uint32_t grid, hipStream_t s);
 void launch_sort(const Args &a, uint32_t grid, uint32_t gather_grid, hipStream_t s);
 void launch_end(const Args &a, hipStream_t s);
+// The per-block inputs of a single submitted block (channel flags, command
+// count, command offset 0, block index 0), written from kernel arguments.
+void launch_stage_one(const Args &a, uint32_t chan_flags, uint32_t ncmd, hipStream_t s);
 // Apply a fused block's permutation as a copy (before the pool is read or
 // written in place): gather, then flip to the gathered buffer.
 void launch_settle(const Args &a, uint32_t gather_grid, hipStream_t s);

@@ -27,6 +27,7 @@
 #include <thread>
 
 #include "kst_host.h"
+#include "kst_stage.h"
 
 namespace kst {
 namespace {
@@ -504,6 +505,15 @@ public:
     // Admission and the build of what it admits (src/driver.rs:556-594).
     void admit(uint64_t live64, uint64_t stolen_before) {
         const auto t0 = Clock::now();
+        const size_t take = admit_plan(live64, stolen_before);
+        if (spawn_buf_.size() < take) spawn_buf_.resize(take);  // grows only: no refill
+        spawn_n_ = materialise(spawn_buf_.data(), take);
+        t_admit_ += Clock::now() - t0;
+    }
+
+    // Admission proper: which candidates, in which order, and their counters.
+    // Returns how many of the (reordered) candidates to build.
+    size_t admit_plan(uint64_t live64, uint64_t stolen_before) {
         const uint32_t live = (uint32_t)live64;
         const size_t want = f_.cands.size();
         const size_t take = admit_take(cfg_, live, (uint32_t)want);
@@ -527,10 +537,8 @@ public:
         }
         stats_.last_want_energy = want_e;
         stats_.last_take_energy = take_e;
-        spawn_buf_.clear();
-        materialise(std::min(take, f_.cands.size()));
         stats_.voices_spawned += want;
-        t_admit_ += Clock::now() - t0;
+        return std::min(take, f_.cands.size());
     }
 
     // After the render (src/driver.rs:596-641). Returns whether there is more.
@@ -598,9 +606,16 @@ public:
             kst_block_stats st{};
             be(kst_stats(b, &st));
             t_backend_ += Clock::now() - tb;
-            admit(st.active_voices, st.stolen);
+            // Admission builds the admitted layers straight into the backend's
+            // pinned command staging (what kst_spawn would copy them into).
+            const auto ta = Clock::now();
+            const size_t take = admit_plan(st.active_voices, st.stolen);
+            kst_spawn_cmd *cmds = nullptr;
+            be(kst::spawn_reserve(b, take, &cmds));
+            const size_t built = materialise(cmds, take);
+            t_admit_ += Clock::now() - ta;
             tb = Clock::now();
-            be(kst_spawn(b, spawn_buf_.data(), spawn_buf_.size()));
+            be(kst::spawn_commit(b, built));
             be(kst_stats(b, &st));
             const uint64_t stolen_after = st.stolen;
             be(kst_render_submit(b));
@@ -641,7 +656,8 @@ public:
         flags[1] = f_.gain;
         flags[2] = f_.variant;
     }
-    const std::vector<kst_spawn_cmd> &spawned() const { return spawn_buf_; }
+    const kst_spawn_cmd *spawned() const { return spawn_buf_.data(); }
+    size_t spawned_count() const { return spawn_n_; }
     const kst_driver_stats &stats() const { return stats_; }
     uint32_t block_frames() const { return cfg_.block_frames; }
 
@@ -718,7 +734,8 @@ private:
         bool src_more = true;  // the file or the deferred queue had more after it
     };
     alignas(128) Front f_;
-    std::vector<kst_spawn_cmd> spawn_buf_;
+    std::vector<kst_spawn_cmd> spawn_buf_;  // the split entry points' spawn list
+    size_t spawn_n_ = 0;
     kst_driver_stats stats_{};
     kst_limiter *limiter_ = nullptr;
     uint64_t block_index_ = 0, tail_blocks_left_ = 0, stolen_before_ = 0;
@@ -802,20 +819,24 @@ private:
         }
     }
 
-    void materialise(size_t take) {  // src/driver.rs:751-783
+    // Builds the first `take` candidates into out[0..], skipping a layer the
+    // bank cannot build (src/driver.rs:751-783); returns how many it wrote.
+    size_t materialise(kst_spawn_cmd *out, size_t take) {
+        size_t n = 0;
         for (size_t i = 0; i < take; ++i) {
             const Cand &c = f_.cands[i];
             if (c.note == DEFERRED) {
-                spawn_buf_.push_back(f_.deferred_now[c.region]);
+                out[n++] = f_.deferred_now[c.region];
                 continue;
             }
             const Note &nt = f_.notes[c.note];
             Layer v;
             if (!bank_.build_layer(c.region, nt.key, nt.vel, v)) continue;
             const uint32_t start_rel = v.delay_frames == 0 ? nt.rel : nt.rel + v.delay_frames;
-            spawn_buf_.push_back(make_cmd(v, nt.variant, GateTable::slot(nt.ch, nt.key), nt.ordinal,
-                                          start_rel, f_.first_id + c.id_off));
+            out[n++] = make_cmd(v, nt.variant, GateTable::slot(nt.ch, nt.key), nt.ordinal, start_rel,
+                                f_.first_id + c.id_off);
         }
+        return n;
     }
 
     void handle_event(const kst_midi_event &ev, uint32_t rel, uint64_t block_start) {
@@ -1332,8 +1353,8 @@ int kst_driver_admit(kst_driver *d, uint64_t live, uint64_t stolen_before,
                      const kst_spawn_cmd **cmds, size_t *n) {
     if (!d || !cmds || !n) return fail(KST_EINVAL, "null argument");
     d->d->admit(live, stolen_before);
-    *cmds = d->d->spawned().data();
-    *n = d->d->spawned().size();
+    *cmds = d->d->spawned();
+    *n = d->d->spawned_count();
     return KST_OK;
 }
 

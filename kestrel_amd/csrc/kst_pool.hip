@@ -983,6 +983,14 @@ __device__ __forceinline__ void end_body(const Args &a) {
 }
 __global__ void k_end(Args a) { end_body(a); }
 
+// A single submitted block's per-block inputs, from kernel arguments.
+__global__ void k_stage_one(Args a, uint32_t chan_flags, uint32_t ncmd) {
+    const_cast<uint32_t *>(a.chan_flags)[0] = chan_flags;
+    const_cast<uint32_t *>(a.cmd_counts)[0] = ncmd;
+    const_cast<uint32_t *>(a.cmd_offs)[0] = 0;
+    a.st->block = 0;
+}
+
 // After the gather of a settle: the gathered buffer becomes the live pool, in
 // order, with no permutation pending.
 __global__ void k_settle(Args a) {
@@ -1025,6 +1033,9 @@ void launch_sort(const Args &a, uint32_t grid, uint32_t gather_grid, hipStream_t
 }
 
 void launch_end(const Args &a, hipStream_t s) { k_end<<<1, a.fused ? 256 : 1, 0, s>>>(a); }
+void launch_stage_one(const Args &a, uint32_t chan_flags, uint32_t ncmd, hipStream_t s) {
+    k_stage_one<<<1, 1, 0, s>>>(a, chan_flags, ncmd);
+}
 
 void launch_settle(const Args &a, uint32_t gather_grid, hipStream_t s) {
     k_gather<<<gather_grid, 256, 0, s>>>(a);
