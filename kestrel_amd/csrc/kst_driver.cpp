@@ -345,19 +345,23 @@ public:
     // params variants wait to be uploaded. The block was usually begun by the
     // event thread while the caller finished the one before; its results are
     // taken over here and that thread starts on the block after.
-    size_t begin_block() {
+    // `start_next`: begin the block after on the event thread now (next_block
+    // passes false when it will begin that block itself, inline).
+    size_t begin_block(bool start_next = true) {
         if (ahead_)
             join_ahead();  // rethrows what the event side threw
+        else if (ready_)
+            ready_ = false;  // begun inline during the last block's render
         else
             begin_at();
         publish();
-        start_ahead();
+        if (start_next) start_ahead();
         return f_.uploads.size();
     }
 
     // The event side of the next block into the back buffers. Runs on the
-    // event thread, or on the caller's before the first block; it touches only
-    // state that admission, the render and the post-render step never do.
+    // event thread or on the caller's; it touches only state that admission,
+    // the render and the post-render step never do.
     void begin_at() {
         const auto t0 = Clock::now();
         const uint64_t F = cfg_.block_frames;
@@ -588,8 +592,10 @@ public:
         if (n != (size_t)cfg_.block_frames * 2)
             return fail(KST_EINVAL, "output block is " + std::to_string(n) + " samples, expected " +
                                         std::to_string(cfg_.block_frames * 2));
+        const auto t_block = Clock::now();
+        const bool inl = pace_inline_;
         try {
-            begin_block();
+            begin_block(!inl);
         } catch (const std::exception &e) {
             return fail(KST_EINVAL, e.what());
         }
@@ -620,29 +626,69 @@ public:
             const uint64_t stolen_after = st.stolen;
             be(kst_render_submit(b));
             t_backend_ += Clock::now() - tb;
+            if (inl) {  // the next block's event side while the device renders
+                begin_at();
+                ready_ = true;
+            }
             tb = Clock::now();
             be(kst_render_collect(b, out, n));
             be(kst_stats(b, &st));
             t_wait_ += Clock::now() - tb;
             ++timed_blocks_;
-            return end_at(out, n, stolen_after, st, f_.src_more);
+            const int more = end_at(out, n, stolen_after, st, f_.src_more);
+            pace(inl, Clock::now() - t_block);
+            return more;
         } catch (const std::runtime_error &e) {
             return fail(KST_EDEVICE, e.what());
+        }
+    }
+
+    // Where next_block runs the event side: on the event thread (beside
+    // admission, the device and the post-render step) or inline on the
+    // caller's thread while the device renders. Which is faster depends on the
+    // machine -- how the two threads' shared buffers travel between the cores
+    // the scheduler picked -- so next_block times both: a 16-block trial of
+    // each, then the faster for 480 blocks, then a trial of the other against
+    // the chosen one's latest blocks, and so on. The first blocks of a stint
+    // are left out of its mean (the switch itself costs one block's overlap).
+    void pace(bool inl, std::chrono::steady_clock::duration took) {
+        ++inline_blocks_[inl];
+        const uint32_t k = PACE_TRIAL - stint_left_;  // position in the last PACE_TRIAL
+        if (stint_left_ <= PACE_TRIAL && k >= PACE_SKIP) {
+            stint_sum_ += std::chrono::duration<double>(took).count();
+            ++stint_n_;
+        }
+        if (--stint_left_ > 0) return;
+        pace_mean_[inl] = stint_n_ ? stint_sum_ / stint_n_ : 0.0;
+        stint_sum_ = 0.0;
+        stint_n_ = 0;
+        if (pace_trial_ && pace_mean_[!inl] == 0.0) {  // the other has no trial yet
+            pace_inline_ = !inl;
+            stint_left_ = PACE_TRIAL;
+        } else if (pace_trial_) {  // a trial ended: keep the faster
+            pace_inline_ = pace_mean_[1] < pace_mean_[0];
+            pace_trial_ = false;
+            stint_left_ = PACE_STINT;
+        } else {  // a stint ended: try the other
+            pace_inline_ = !inl;
+            pace_trial_ = true;
+            stint_left_ = PACE_TRIAL;
         }
     }
 
     size_t timings(kst_timing *out, size_t cap) const {
         if (timed_blocks_ == 0) return 0;
         const double k = 1e-6 / (double)timed_blocks_;
-        const kst_timing t[6] = {
+        const kst_timing t[7] = {
             {"events_rows", std::chrono::duration<double, std::nano>(t_events_).count() * k},
             {"events_wait", std::chrono::duration<double, std::nano>(t_join_).count() * k},
             {"admission_build", std::chrono::duration<double, std::nano>(t_admit_).count() * k},
             {"post_render", std::chrono::duration<double, std::nano>(t_post_).count() * k},
             {"backend_calls", std::chrono::duration<double, std::nano>(t_backend_).count() * k},
-            {"device_wait", std::chrono::duration<double, std::nano>(t_wait_).count() * k}};
-        for (size_t i = 0; i < 6 && i < cap; ++i) out[i] = t[i];
-        return 6;
+            {"device_wait", std::chrono::duration<double, std::nano>(t_wait_).count() * k},
+            {"inline_share", (double)inline_blocks_[1] / (double)timed_blocks_}};
+        for (size_t i = 0; i < 7 && i < cap; ++i) out[i] = t[i];
+        return 7;
     }
 
     // The block begin_block took over: valid until the next begin_block.
@@ -741,6 +787,12 @@ private:
     uint64_t block_index_ = 0, tail_blocks_left_ = 0, stolen_before_ = 0;
     Clock::duration t_events_{}, t_admit_{}, t_post_{}, t_backend_{}, t_wait_{}, t_join_{};
     uint64_t timed_blocks_ = 0;
+    // next_block's choice of where the event side runs (see pace()).
+    static constexpr uint32_t PACE_TRIAL = 16, PACE_SKIP = 2, PACE_STINT = 480;
+    bool pace_inline_ = false, pace_trial_ = true, ready_ = false;
+    uint32_t stint_left_ = PACE_TRIAL, stint_n_ = 0;
+    double stint_sum_ = 0.0, pace_mean_[2] = {0.0, 0.0};
+    uint64_t inline_blocks_[2] = {0, 0};
 
     // The event thread and its one-job handshake: job_ is set while it
     // begins a block, err_ holds what that threw.
