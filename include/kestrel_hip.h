@@ -408,11 +408,12 @@ int kst_driver_create(const kst_config *cfg, const kst_driver_config *dcfg,
 /* Driver::next_block, src/driver.rs:434-642, against this library's backend:
  * returns 1 while the file or its tail has more, 0 when done, < 0 on failure.
  * `n` == block_frames * 2. The driver's event side (events, rows, variants,
- * candidates) runs one block ahead on a thread of its own, so the next block's
- * events are drained while this one is admitted, rendered and limited (nothing
- * of that depends on them); what each call returns and what
- * kst_driver_get_stats reports after it are those of the reference's
- * sequential loop. One caller thread at a time per driver. */
+ * candidates) runs one block ahead -- on a thread of its own while this block
+ * is admitted, rendered and limited, or on the caller's thread while the
+ * device renders, whichever these calls have timed faster (nothing of this
+ * block depends on it); what each call returns and what kst_driver_get_stats
+ * reports after it are those of the reference's sequential loop. One caller
+ * thread at a time per driver. */
 int kst_driver_next_block(kst_driver *d, kst_backend *b, float *out, size_t n);
 /* The same block in the steps a caller with another backend drives itself:
  * begin (events, rows, variants; returns how many variants to upload), then
@@ -431,10 +432,12 @@ int kst_driver_admit(kst_driver *d, uint64_t live, uint64_t stolen_before,
 int kst_driver_end_block(kst_driver *d, float *out, size_t n, uint64_t stolen_after_spawn,
                          const kst_block_stats *after_render);
 int kst_driver_get_stats(const kst_driver *d, kst_driver_stats *out);
-/* Host time per block of kst_driver_next_block, by phase (events and rows on
- * the event thread, the caller's wait for that thread, admission and voice
- * build, post-render step, the backend calls, the wait for the device);
- * writes up to `cap` entries and returns how many there are.
+/* Host time per block of kst_driver_next_block, by phase (events and rows,
+ * the caller's wait for the event thread, admission and voice build,
+ * post-render step, the backend calls, the wait for the device), then
+ * "inline_share": the fraction of blocks (not milliseconds) whose event side
+ * ran on the caller's thread; writes up to `cap` entries and returns how many
+ * there are.
  * No counterpart in the reference. */
 size_t kst_driver_timings(const kst_driver *d, kst_timing *out, size_t cap);
 void kst_driver_destroy(kst_driver *d);
