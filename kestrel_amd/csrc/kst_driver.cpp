@@ -399,26 +399,25 @@ public:
         }
 
         // Events of this block; a tempo change applies from its own tick.
+        // The source hands out runs of events; an event past the block stays
+        // at the cursor for the block it lands in.
         for (;;) {
-            kst_midi_event ev;
-            if (have_pending_) {
-                ev = pending_;
-                have_pending_ = false;
-            } else if (!src_->next(ev)) {
+            if (sp_n_ == 0 && (sp_n_ = src_->span(&sp_)) == 0) {
                 stream_done_ = true;
                 break;
             }
+            const kst_midi_event &ev = *sp_;
             if (ev.kind == KST_EV_TEMPO) {
                 clock_.set_tempo(ev.tick, (uint32_t)ev.value);
                 ++counts_.events;
+                ++sp_;
+                --sp_n_;
                 continue;
             }
             const uint64_t frame = sat_u64(clock_.frame_at(ev.tick));
-            if (frame >= block_end) {
-                pending_ = ev;
-                have_pending_ = true;
-                break;
-            }
+            if (frame >= block_end) break;
+            ++sp_;
+            --sp_n_;
             const uint32_t rel = std::min<uint32_t>(
                 (uint32_t)(frame > block_start ? frame - block_start : 0), cfg_.block_frames - 1);
             ++counts_.events;
@@ -749,8 +748,9 @@ private:
     uint64_t variant_clock_ = 0;
     std::vector<std::pair<uint32_t, ParamMod>> pending_variants_;
     uint64_t next_note_id_ = 1;
-    kst_midi_event pending_{};
-    bool have_pending_ = false, stream_done_ = false, end_sent_ = false;
+    const kst_midi_event *sp_ = nullptr;  // cursor into the source's current run
+    size_t sp_n_ = 0;
+    bool stream_done_ = false, end_sent_ = false;
     std::vector<Preview> preview_;
     std::vector<std::pair<uint64_t, kst_spawn_cmd>> deferred_;
     uint64_t begin_index_ = 0;  // the block begin_at drains next

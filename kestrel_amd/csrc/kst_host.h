@@ -133,7 +133,17 @@ public:
     virtual ~EventSource() = default;
     // Next event in tick order; false at the end.
     virtual bool next(kst_midi_event &ev) = 0;
+    // The next run of events in tick order, [*first, *first + n), valid until
+    // the next call; 0 at the end. By default one event at a time.
+    virtual size_t span(const kst_midi_event **first) {
+        if (!next(one_)) return 0;
+        *first = &one_;
+        return 1;
+    }
     kst_division division{};
+
+private:
+    kst_midi_event one_{};
 };
 
 class MidiFile : public EventSource {
@@ -195,6 +205,12 @@ public:
         ev = events_[at_++];
         return true;
     }
+    size_t span(const kst_midi_event **first) override {
+        *first = events_.data() + at_;
+        const size_t n = events_.size() - at_;
+        at_ = events_.size();
+        return n;
+    }
 
 private:
     std::vector<kst_midi_event> events_;
@@ -219,9 +235,11 @@ public:
     explicit Prefetch(std::unique_ptr<EventSource> inner, size_t chunk = 1 << 15, size_t depth = 32);
     ~Prefetch() override;
     bool next(kst_midi_event &ev) override;
+    size_t span(const kst_midi_event **first) override;  // the rest of the current chunk
 
 private:
     void run();
+    bool refill();  // the next chunk into cur_; false at the end
     std::unique_ptr<EventSource> inner_;
     size_t chunk_, depth_;
     std::mutex m_;

@@ -369,11 +369,7 @@ void Prefetch::run() {
     }
 }
 
-bool Prefetch::next(kst_midi_event &ev) {
-    if (at_ < cur_.size()) {
-        ev = cur_[at_++];
-        return true;
-    }
+bool Prefetch::refill() {
     std::unique_lock<std::mutex> g(m_);
     cv_.wait(g, [&] { return !q_.empty() || done_; });
     if (q_.empty()) return false;
@@ -383,8 +379,21 @@ bool Prefetch::next(kst_midi_event &ev) {
     g.unlock();
     cv_.notify_all();
     at_ = 0;
+    return true;
+}
+
+bool Prefetch::next(kst_midi_event &ev) {
+    if (at_ >= cur_.size() && !refill()) return false;
     ev = cur_[at_++];
     return true;
+}
+
+size_t Prefetch::span(const kst_midi_event **first) {
+    if (at_ >= cur_.size() && !refill()) return 0;
+    *first = cur_.data() + at_;
+    const size_t n = cur_.size() - at_;
+    at_ = cur_.size();
+    return n;
 }
 
 TempoClock::TempoClock(const kst_division &d, uint32_t sample_rate)
