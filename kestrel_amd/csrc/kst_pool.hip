@@ -800,7 +800,9 @@ __global__ void __launch_bounds__(256) kb_sort_scan(const Args *__restrict__ aa,
 // lanes in order within a wave (match masks built from ballots). A digit's
 // first lane in each wave publishes the wave's count for that digit and
 // clears it again after the round, so the per-wave table stays zero between
-// rounds without a full sweep.
+// rounds without a full sweep. Ranked items are first placed in LDS in digit
+// order (a stable local sort of the tile), then written out by consecutive
+// threads, so each digit's run within the tile leaves in adjacent addresses.
 __device__ __forceinline__ void sort_scatter_body(const Args &a, uint32_t p) {
     if (p >= a.sort_passes) return;  // a batch runs the most passes any member needs
     DevState *st = a.st;
@@ -818,6 +820,9 @@ __device__ __forceinline__ void sort_scatter_body(const Args &a, uint32_t p) {
     __shared__ uint32_t cnt[SORT_MAX_RADIX];
     __shared__ uint32_t wc[WAVES][SORT_MAX_RADIX];
     __shared__ uint32_t buf[256];
+    __shared__ uint32_t gbase[SORT_MAX_RADIX];  // the tile's first output slot per digit
+    __shared__ uint32_t lbase[SORT_MAX_RADIX];  // the digit's first slot in the local order
+    __shared__ uint32_t lkey[SORT_TILE], lidx[SORT_TILE];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
     // Digit bases: exclusive scan of this pass's digit totals.
@@ -854,8 +859,35 @@ __device__ __forceinline__ void sort_scatter_body(const Args &a, uint32_t p) {
             kk[k] = valid ? kin[item] : 0u;
             ii[k] = valid ? (p == 0 ? item : iin[item]) : 0u;
         }
-        for (uint32_t d = threadIdx.x; d < radix; d += SORT_TPB)
-            cnt[d] = base[d] + a.sort_hist[(size_t)(p * radix + d) * a.tiles_max + t];
+        for (uint32_t d = threadIdx.x; d < SORT_MAX_RADIX; d += SORT_TPB) {
+            const uint32_t g = d < radix ? base[d] + a.sort_hist[(size_t)(p * radix + d) * a.tiles_max + t] : 0u;
+            cnt[d] = g;
+            gbase[d] = g;
+            lbase[d] = 0;
+        }
+        __syncthreads();
+        // The tile's own digit counts, then their exclusive scan: lbase.
+#pragma unroll
+        for (uint32_t k = 0; k < SORT_IPT; ++k) {
+            const uint32_t item = t * SORT_TILE + k * SORT_TPB + threadIdx.x;
+            if (item < n) atomicAdd(&lbase[(kk[k] >> shift) & mask], 1u);
+        }
+        __syncthreads();
+        {
+            uint32_t v[PER], sum = 0;
+#pragma unroll
+            for (uint32_t j = 0; j < PER; ++j) {
+                v[j] = lbase[threadIdx.x * PER + j];
+                sum += v[j];
+            }
+            uint32_t total;
+            uint32_t run = block_exscan256(sum, buf, &total);
+#pragma unroll
+            for (uint32_t j = 0; j < PER; ++j) {
+                lbase[threadIdx.x * PER + j] = run;
+                run += v[j];
+            }
+        }
         __syncthreads();
 #pragma unroll
         for (uint32_t k = 0; k < SORT_IPT; ++k) {
@@ -875,14 +907,26 @@ __device__ __forceinline__ void sort_scatter_body(const Args &a, uint32_t p) {
             if (valid) {
                 uint32_t off = cnt[digit] + rank;
                 for (uint32_t w = 0; w < wave; ++w) off += wc[w][digit];
-                kout[off] = key;
-                iout[off] = ii[k];
+                const uint32_t lpos = lbase[digit] + (off - gbase[digit]);
+                lkey[lpos] = key;
+                lidx[lpos] = ii[k];
             }
             __syncthreads();
             if (leader) {
                 atomicAdd(&cnt[digit], (uint32_t)__popcll(m));
                 wc[wave][digit] = 0;
             }
+        }
+        __syncthreads();
+        // Out in local order: slot j holds the (j - lbase[d])-th item of digit d.
+        const uint32_t t0 = t * SORT_TILE;
+        const uint32_t items = n - t0 < SORT_TILE ? n - t0 : SORT_TILE;
+        for (uint32_t j = threadIdx.x; j < items; j += SORT_TPB) {
+            const uint32_t key = lkey[j];
+            const uint32_t d = (key >> shift) & mask;
+            const uint32_t off = gbase[d] + (j - lbase[d]);
+            kout[off] = key;
+            iout[off] = lidx[j];
         }
         __syncthreads();
     }
