@@ -626,7 +626,12 @@ public:
             be(kst_render_submit(b));
             t_backend_ += Clock::now() - tb;
             if (inl) {  // the next block's event side while the device renders
-                begin_at();
+                try {
+                    begin_at();
+                } catch (const std::exception &e) {
+                    (void)kst_render_collect(b, out, n);  // leave the backend usable
+                    return fail(KST_EINVAL, e.what());
+                }
                 ready_ = true;
             }
             tb = Clock::now();
