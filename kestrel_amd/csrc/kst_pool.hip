@@ -712,7 +712,8 @@ __device__ __forceinline__ bool key_alive(const Args &a, uint32_t key) {
 
 // Per-tile digit counts of pass p over that pass's input order (the previous
 // pass's output). Each tile's keys are loaded up front; counts go to LDS and
-// then to the tile's column of sort_hist, with no global atomics.
+// then to the tile's row of sort_hist (tile-major: a tile's digits are
+// adjacent), with no global atomics.
 __device__ __forceinline__ void sort_hist_body(const Args &a, uint32_t p) {
     if (p >= a.sort_passes) return;  // a batch runs the most passes any member needs
     DevState *st = a.st;
@@ -738,7 +739,7 @@ __device__ __forceinline__ void sort_hist_body(const Args &a, uint32_t p) {
         }
         __syncthreads();
         for (uint32_t d = threadIdx.x; d < radix; d += SORT_TPB)
-            a.sort_hist[(size_t)(p * radix + d) * a.tiles_max + t] = sh[d];
+            a.sort_hist[((size_t)p * a.tiles_max + t) * radix + d] = sh[d];
         __syncthreads();
     }
 }
@@ -766,32 +767,48 @@ __device__ __forceinline__ uint32_t block_exscan256(uint32_t v, uint32_t *buf, u
     return incl - v;
 }
 
-// One workgroup per (pass, digit): each tile's offset within the digit is the
-// same digit's count in the tiles before it; the digit's total goes to
-// sort_tot for the scatter to place the digits.
+// Each tile's offset within a digit is the same digit's count in the tiles
+// before it; the digit's total goes to sort_tot for the scatter to place the
+// digits. A workgroup takes 64 digits (one per lane, so a wave reads a tile's
+// 64 adjacent counts at once) and splits the tiles into SCAN_LANES contiguous
+// ranges, one per wave: each wave sums its range, the waves' sums are scanned
+// in LDS, and each wave writes its range's offsets from its starting sum.
+constexpr uint32_t SCAN_LANES = 16;
 __device__ __forceinline__ void sort_scan_body(const Args &a, uint32_t p) {
-    if (p >= a.sort_passes || blockIdx.x >= (1u << a.digit_bits)) return;
+    const uint32_t radix = 1u << a.digit_bits;
+    if (p >= a.sort_passes || blockIdx.x * 64 >= radix) return;
     DevState *st = a.st;
     const uint32_t n = st->live_total;
     const uint32_t tiles = (n + SORT_TILE - 1) / SORT_TILE;
-    const uint32_t radix = 1u << a.digit_bits;
-    const uint32_t d = blockIdx.x;
-    __shared__ uint32_t buf[256];
-    uint32_t *row = a.sort_hist + (size_t)(p * radix + d) * a.tiles_max;
-    uint32_t running = 0;
-    for (uint32_t base = 0; base < tiles; base += 256) {
-        const uint32_t t = base + threadIdx.x;
-        const uint32_t v = t < tiles ? row[t] : 0u;
-        uint32_t total;
-        const uint32_t ex = block_exscan256(v, buf, &total);
-        if (t < tiles) row[t] = running + ex;
-        running += total;
+    const uint32_t col = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t d = blockIdx.x * 64 + col;
+    const uint32_t per = (tiles + SCAN_LANES - 1) / SCAN_LANES;
+    const uint32_t t0 = w * per < tiles ? w * per : tiles;
+    const uint32_t t1 = t0 + per < tiles ? t0 + per : tiles;
+    uint32_t *h = a.sort_hist + (size_t)p * a.tiles_max * radix;
+    __shared__ uint32_t part[SCAN_LANES][64];
+    uint32_t sum = 0;
+    if (d < radix) {
+#pragma unroll 8
+        for (uint32_t t = t0; t < t1; ++t) sum += h[(size_t)t * radix + d];
     }
-    if (threadIdx.x == 0) a.sort_tot[p * radix + d] = running;
+    part[w][col] = sum;
+    __syncthreads();
+    uint32_t run = 0;
+    for (uint32_t l = 0; l < w; ++l) run += part[l][col];
+    if (d < radix) {
+#pragma unroll 8
+        for (uint32_t t = t0; t < t1; ++t) {
+            const uint32_t v = h[(size_t)t * radix + d];
+            h[(size_t)t * radix + d] = run;
+            run += v;
+        }
+        if (w == SCAN_LANES - 1) a.sort_tot[p * radix + d] = run;
+    }
 }
-__global__ void __launch_bounds__(256) k_sort_scan(Args a, uint32_t p) { sort_scan_body(a, p); }
+__global__ void __launch_bounds__(1024) k_sort_scan(Args a, uint32_t p) { sort_scan_body(a, p); }
 // The same pass over a batch of renders: blockIdx.z picks the render.
-__global__ void __launch_bounds__(256) kb_sort_scan(const Args *__restrict__ aa, uint32_t p) {
+__global__ void __launch_bounds__(1024) kb_sort_scan(const Args *__restrict__ aa, uint32_t p) {
     sort_scan_body(aa[blockIdx.z], p);
 }
 
@@ -860,7 +877,7 @@ __device__ __forceinline__ void sort_scatter_body(const Args &a, uint32_t p) {
             ii[k] = valid ? (p == 0 ? item : iin[item]) : 0u;
         }
         for (uint32_t d = threadIdx.x; d < SORT_MAX_RADIX; d += SORT_TPB) {
-            const uint32_t g = d < radix ? base[d] + a.sort_hist[(size_t)(p * radix + d) * a.tiles_max + t] : 0u;
+            const uint32_t g = d < radix ? base[d] + a.sort_hist[((size_t)p * a.tiles_max + t) * radix + d] : 0u;
             cnt[d] = g;
             gbase[d] = g;
             lbase[d] = 0;
@@ -1070,7 +1087,7 @@ void launch_sort(const Args &a, uint32_t grid, uint32_t gather_grid, hipStream_t
     const uint32_t radix = 1u << a.digit_bits;
     for (uint32_t p = 0; p < a.sort_passes; ++p) {
         k_sort_hist<<<grid, SORT_TPB, 0, s>>>(a, p);
-        k_sort_scan<<<radix, 256, 0, s>>>(a, p);
+        k_sort_scan<<<(radix + 63) / 64, 64 * SCAN_LANES, 0, s>>>(a, p);
         k_sort_scatter<<<grid, SORT_TPB, 0, s>>>(a, p);
     }
     if (!a.fused) k_gather<<<gather_grid, 256, 0, s>>>(a);
@@ -1109,7 +1126,7 @@ void launch_batch_sort(const Args *aa, uint32_t nr, uint32_t grid, uint32_t gath
                        uint32_t passes, uint32_t max_radix, hipStream_t s) {
     for (uint32_t p = 0; p < passes; ++p) {
         kb_sort_hist<<<dim3(grid, 1, nr), SORT_TPB, 0, s>>>(aa, p);
-        kb_sort_scan<<<dim3(max_radix, 1, nr), 256, 0, s>>>(aa, p);
+        kb_sort_scan<<<dim3((max_radix + 63) / 64, 1, nr), 64 * SCAN_LANES, 0, s>>>(aa, p);
         kb_sort_scatter<<<dim3(grid, 1, nr), SORT_TPB, 0, s>>>(aa, p);
     }
     kb_gather<<<dim3(gather_grid, 1, nr), 256, 0, s>>>(aa);
