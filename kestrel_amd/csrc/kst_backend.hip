@@ -428,7 +428,9 @@ struct kst_backend {
         hipStream_t s = stream;
         if (timed) (void)hipEventRecord(ev[0], s);
         if (may_steal && !pipelined) launch_sel(n_live, false, s);
-        launch_begin(a, s);
+        // Pipelined, the last block's k_end began this one (the run's first
+        // block is begun before its loop).
+        if (!pipelined) launch_begin(a, s);
         if (timed) (void)hipEventRecord(ev[1], s);
         if (spawn) launch_spawn(a, per_voice_grid(n_total), s);
         if (timed) (void)hipEventRecord(ev[2], s);
@@ -449,6 +451,7 @@ struct kst_backend {
             launch_reduce_plain(a, s);
         if (timed) (void)hipEventRecord(ev[4], s);
         launch_sort(a, sort_grid(n_total), per_voice_grid(n_total), s);
+        if (pipelined) a.end_begins = 1;
         launch_end(a, s);
         if (pipelined) (void)hipStreamWaitEvent(s, ev_selected, 0);
         if (timed) (void)hipEventRecord(ev[5], s);
@@ -948,6 +951,7 @@ int kst_run_staged(kst_backend *b, int use_graph) {
     const bool pipe = b->fast && b->cfg.max_voices >= PIPELINE_MIN_VOICES;
     if (pipe) {
         b->launch_sel(b->cfg.max_voices, false, s);
+        launch_begin(b->args(1), s);  // each block's k_end begins the next
         HIPCHK(hipGetLastError());
     }
     if (use_graph) {

@@ -181,6 +181,9 @@ struct Args {
     // Select the victims of the block after the last render (staged runs,
     // on the side stream) rather than of the block about to run.
     uint32_t sel_ahead;
+    // k_end also begins the next block (staged runs with the selection ahead,
+    // where nothing else runs between the two): one launch fewer per block.
+    uint32_t end_begins;
     // Fused: the top digit of the last sort pass at which dead keys begin, when
     // that boundary falls on a digit (the survivor count is then that digit's
     // base); ~0u when it does not.

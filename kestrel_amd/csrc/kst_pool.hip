@@ -1041,6 +1041,9 @@ __device__ __forceinline__ void end_body(const Args &a) {
     st->parity ^= 1;
     st->perm_valid = a.fused;
     st->block += 1;
+    // Begin the next block here when there is one (after the last, the state
+    // keeps this block's counts for a settle's gather).
+    if (a.end_begins && st->block < a.nblocks) begin_body(a);  // thread 0, as begin_body needs
 }
 __global__ void k_end(Args a) { end_body(a); }
 
