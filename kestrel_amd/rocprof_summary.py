@@ -2,7 +2,8 @@
 
     python -m kestrel_amd.rocprof_summary trace DIR [--blocks K]
         per-kernel mean duration over the last K blocks of a --kernel-trace run
-        (blocks start at each k_begin dispatch), plus the whole-run --stats table;
+        (blocks start at each k_spawn dispatch, else at each k_begin), plus the
+        whole-run --stats table;
     python -m kestrel_amd.rocprof_summary pmc DIR --kernel NAME [--last K]
         mean of each counter over the last K dispatches of one kernel;
     python -m kestrel_amd.rocprof_summary traffic --fetch DIR --write DIR
@@ -49,7 +50,11 @@ def _rows(path: str):
 def trace(d: str, blocks: int) -> str:
     rows = _rows(_one(d, "kernel_trace.csv"))
     rows.sort(key=lambda r: int(r["Start_Timestamp"]))
-    begins = [i for i, r in enumerate(rows) if r["Kernel_Name"].startswith("k_begin")]
+    # A block's chain opens with k_spawn (staged runs begin each block inside
+    # the previous one's k_end); without spawns, with k_begin.
+    begins = [i for i, r in enumerate(rows) if r["Kernel_Name"].startswith("k_spawn")]
+    if len(begins) < blocks:
+        begins = [i for i, r in enumerate(rows) if r["Kernel_Name"].startswith("k_begin")]
     out = [f"### Last {blocks} blocks (from {os.path.basename(_one(d, 'kernel_trace.csv'))})", "",
            "| kernel | launches | mean us | share of block |", "|---|---:|---:|---:|"]
     if len(begins) >= blocks:
