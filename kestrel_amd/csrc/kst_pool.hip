@@ -913,9 +913,17 @@ __device__ __forceinline__ void sort_scatter_body(const Args &a, uint32_t p) {
             const uint32_t key = kk[k];
             const uint32_t digit = (key >> shift) & mask;
             uint64_t m = __ballot(valid);
-            for (uint32_t bit = 0; bit < a.digit_bits; ++bit) {
-                const uint64_t bb = __ballot(valid && ((digit >> bit) & 1));
-                m &= ((digit >> bit) & 1) ? bb : ~bb;
+            if (a.digit_bits == SORT_MAX_BITS) {  // the usual width, unrolled
+#pragma unroll
+                for (uint32_t bit = 0; bit < SORT_MAX_BITS; ++bit) {
+                    const uint64_t bb = __ballot(valid && ((digit >> bit) & 1));
+                    m &= ((digit >> bit) & 1) ? bb : ~bb;
+                }
+            } else {
+                for (uint32_t bit = 0; bit < a.digit_bits; ++bit) {
+                    const uint64_t bb = __ballot(valid && ((digit >> bit) & 1));
+                    m &= ((digit >> bit) & 1) ? bb : ~bb;
+                }
             }
             const uint32_t rank = __popcll(m & lt);
             const bool leader = valid && rank == 0;
