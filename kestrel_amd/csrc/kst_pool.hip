@@ -931,7 +931,9 @@ __device__ __forceinline__ void sort_scatter_body(const Args &a, uint32_t p) {
             __syncthreads();
             if (valid) {
                 uint32_t off = cnt[digit] + rank;
-                for (uint32_t w = 0; w < wave; ++w) off += wc[w][digit];
+#pragma unroll
+                for (uint32_t w = 0; w < WAVES - 1; ++w)
+                    if (w < wave) off += wc[w][digit];
                 const uint32_t lpos = lbase[digit] + (off - gbase[digit]);
                 lkey[lpos] = key;
                 lidx[lpos] = ii[k];
