@@ -769,24 +769,24 @@ __device__ __forceinline__ uint32_t block_exscan256(uint32_t v, uint32_t *buf, u
 
 // Each tile's offset within a digit is the same digit's count in the tiles
 // before it; the digit's total goes to sort_tot for the scatter to place the
-// digits. A workgroup takes 64 digits (one per lane, so a wave reads a tile's
-// 64 adjacent counts at once) and splits the tiles into SCAN_LANES contiguous
-// ranges, one per wave: each wave sums its range, the waves' sums are scanned
-// in LDS, and each wave writes its range's offsets from its starting sum.
-constexpr uint32_t SCAN_LANES = 16;
+// digits. A workgroup takes SCAN_DIGITS adjacent digits (a half-wave reads a
+// tile's 32 adjacent counts at once) and splits the tiles into SCAN_LANES
+// contiguous ranges, one per half-wave: each sums its range, the ranges' sums
+// are scanned in LDS, and each range's offsets are written from its start.
+constexpr uint32_t SCAN_DIGITS = 32, SCAN_LANES = 32;
 __device__ __forceinline__ void sort_scan_body(const Args &a, uint32_t p) {
     const uint32_t radix = 1u << a.digit_bits;
-    if (p >= a.sort_passes || blockIdx.x * 64 >= radix) return;
+    if (p >= a.sort_passes || blockIdx.x * SCAN_DIGITS >= radix) return;
     DevState *st = a.st;
     const uint32_t n = st->live_total;
     const uint32_t tiles = (n + SORT_TILE - 1) / SORT_TILE;
-    const uint32_t col = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const uint32_t d = blockIdx.x * 64 + col;
+    const uint32_t col = threadIdx.x % SCAN_DIGITS, w = threadIdx.x / SCAN_DIGITS;
+    const uint32_t d = blockIdx.x * SCAN_DIGITS + col;
     const uint32_t per = (tiles + SCAN_LANES - 1) / SCAN_LANES;
     const uint32_t t0 = w * per < tiles ? w * per : tiles;
     const uint32_t t1 = t0 + per < tiles ? t0 + per : tiles;
     uint32_t *h = a.sort_hist + (size_t)p * a.tiles_max * radix;
-    __shared__ uint32_t part[SCAN_LANES][64];
+    __shared__ uint32_t part[SCAN_LANES][SCAN_DIGITS];
     uint32_t sum = 0;
     if (d < radix) {
 #pragma unroll 8
@@ -1100,7 +1100,7 @@ void launch_sort(const Args &a, uint32_t grid, uint32_t gather_grid, hipStream_t
     const uint32_t radix = 1u << a.digit_bits;
     for (uint32_t p = 0; p < a.sort_passes; ++p) {
         k_sort_hist<<<grid, SORT_TPB, 0, s>>>(a, p);
-        k_sort_scan<<<(radix + 63) / 64, 64 * SCAN_LANES, 0, s>>>(a, p);
+        k_sort_scan<<<(radix + SCAN_DIGITS - 1) / SCAN_DIGITS, SCAN_DIGITS * SCAN_LANES, 0, s>>>(a, p);
         k_sort_scatter<<<grid, SORT_TPB, 0, s>>>(a, p);
     }
     if (!a.fused) k_gather<<<gather_grid, 256, 0, s>>>(a);
@@ -1139,7 +1139,8 @@ void launch_batch_sort(const Args *aa, uint32_t nr, uint32_t grid, uint32_t gath
                        uint32_t passes, uint32_t max_radix, hipStream_t s) {
     for (uint32_t p = 0; p < passes; ++p) {
         kb_sort_hist<<<dim3(grid, 1, nr), SORT_TPB, 0, s>>>(aa, p);
-        kb_sort_scan<<<dim3((max_radix + 63) / 64, 1, nr), 64 * SCAN_LANES, 0, s>>>(aa, p);
+        kb_sort_scan<<<dim3((max_radix + SCAN_DIGITS - 1) / SCAN_DIGITS, 1, nr),
+                       SCAN_DIGITS * SCAN_LANES, 0, s>>>(aa, p);
         kb_sort_scatter<<<dim3(grid, 1, nr), SORT_TPB, 0, s>>>(aa, p);
     }
     kb_gather<<<dim3(gather_grid, 1, nr), 256, 0, s>>>(aa);
