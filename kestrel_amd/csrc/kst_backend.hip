@@ -228,6 +228,9 @@ struct kst_backend {
 
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    // The captured chain holds its kernels' arguments by value, the staged
+    // block count among them: a run of another count captures again.
+    uint32_t graph_staged = 0;
     // Staged runs select the next block's victims on a side stream while the
     // current block reduces and sorts (default priority, like the main stream).
     hipStream_t side = nullptr;
@@ -955,16 +958,13 @@ int kst_run_staged(kst_backend *b, int use_graph) {
         HIPCHK(hipGetLastError());
     }
     if (use_graph) {
-        if (b->graph_exec && b->graph_chan != chan) {
-            (void)hipGraphExecDestroy(b->graph_exec);
-            (void)hipGraphDestroy(b->graph);
-            b->graph_exec = nullptr;
-            b->graph = nullptr;
-        }
+        if (b->graph_exec && (b->graph_chan != chan || b->graph_staged != b->staged))
+            b->drop_graph();
         if (!b->graph_exec) {
             HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             b->launch_block(b->cfg.max_voices, n, true, true, chan, nullptr, pipe);
             b->graph_chan = chan;
+            b->graph_staged = b->staged;
             hipError_t e = hipStreamEndCapture(s, &b->graph);
             if (e != hipSuccess) return fail(KST_EDEVICE, "graph capture: %s", hipGetErrorString(e));
             HIPCHK(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));

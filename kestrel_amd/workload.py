@@ -145,17 +145,6 @@ class Bank:
     def stereo(self) -> bool:
         return self.smp_start_r is not None
 
-    def params_f64(self, cfg: Config) -> np.ndarray:
-        """The same design carried out in f64 by the oracle (precision form)."""
-        from oracle import oracle
-        out = np.zeros(len(self.design), oracle.PARAMS64_DTYPE)
-        for i, d in enumerate(self.design):
-            out[i] = oracle.make_params_f64(
-                d["attack_s"], d["hold_s"], d["decay_s"], d["sustain"], d["release_s"],
-                d["fc_cents"], d["q_cb"], float(cfg.sample_rate), cfg.decay_curve == 0,
-                cfg.release_curve == 0, float(F32(cfg.env_floor)), bool(d["filter"]))
-        return out
-
     @property
     def n_regions(self) -> int:
         return len(self.smp_start)

@@ -29,6 +29,23 @@
 #include <vector>
 
 namespace {
+// Runs fn(lo, hi) over [0, n) split into `threads` contiguous ranges. Only
+// independent per-index work goes through here, so the result does not depend
+// on the thread count.
+template <typename Fn>
+void par_for(size_t n, int threads, Fn fn) {
+    if (threads <= 1 || n < 65536) {
+        fn((size_t)0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int k = 0; k < threads; ++k)
+        th.emplace_back([=]() { fn(n * k / threads, n * (k + 1) / threads); });
+    for (auto &t : th) t.join();
+}
+}  // namespace
+
+namespace {
 
 constexpr uint32_t ENV_ATTACK = 0, ENV_DECAY = 1, ENV_SUSTAIN = 2, ENV_RELEASE = 3,
                    ENV_DEAD = 4;
@@ -172,12 +189,20 @@ struct Pool {
     }
 
     // Reorder by a permutation (sorted mode): new[j] = old[perm[j]].
-    void permute(const std::vector<uint32_t> &perm) {
+    void permute(const std::vector<uint32_t> &perm, int threads) {
+        std::vector<std::thread> th;
         for_each_field([&](auto &v) {
-            auto old = v;
-            v.resize(perm.size());
-            for (size_t j = 0; j < perm.size(); ++j) v[j] = old[perm[j]];
+            auto job = [&v, &perm]() {
+                auto old = v;
+                v.resize(perm.size());
+                for (size_t j = 0; j < perm.size(); ++j) v[j] = old[perm[j]];
+            };
+            if (threads > 1 && perm.size() >= 65536)
+                th.emplace_back(job);
+            else
+                job();
         });
+        for (auto &t : th) t.join();
     }
 };
 
@@ -209,6 +234,24 @@ namespace {
 // ---------------------------------------------------------------------------
 // render, templated on the arithmetic precision
 // ---------------------------------------------------------------------------
+
+// The slot indices in stable order of their 32-bit keys: an LSD counting sort,
+// 16 bits per pass. A stable sort by key has one result, so this is the order
+// std::stable_sort gives, in linear time (pools of 67M voices).
+std::vector<uint32_t> stable_order(const std::vector<uint32_t> &key) {
+    const size_t n = key.size();
+    std::vector<uint32_t> a(n), b(n);
+    std::iota(a.begin(), a.end(), 0u);
+    std::vector<size_t> cnt(65537);
+    for (int shift = 0; shift < 32; shift += 16) {
+        std::fill(cnt.begin(), cnt.end(), 0);
+        for (size_t i = 0; i < n; ++i) ++cnt[((key[a[i]] >> shift) & 0xFFFFu) + 1];
+        for (size_t d = 0; d < 65536; ++d) cnt[d + 1] += cnt[d];
+        for (size_t i = 0; i < n; ++i) b[cnt[(key[a[i]] >> shift) & 0xFFFFu]++] = a[i];
+        a.swap(b);
+    }
+    return a;
+}
 
 template <typename R>
 struct Renderer {
@@ -518,13 +561,18 @@ struct Renderer {
                 // The victim set is the `need` smallest keys; keys are a total
                 // order with no ties, so any selection algorithm names the same
                 // set as select_nth_unstable_by_key.
-                std::vector<uint32_t> order(live);
-                std::iota(order.begin(), order.end(), 0u);
-                std::nth_element(order.begin(), order.begin() + (need - 1), order.end(),
-                                 [&](uint32_t a, uint32_t b) { return steal_key(a) < steal_key(b); });
+                // The threshold is the need-th smallest key; exactly `need`
+                // keys are at or below it.
+                std::vector<uint64_t> keys(live);
+                par_for(live, s.threads, [&](size_t lo, size_t hi) {
+                    for (size_t i = lo; i < hi; ++i) keys[i] = steal_key(i);
+                });
+                std::vector<uint64_t> tmp(keys);
+                std::nth_element(tmp.begin(), tmp.begin() + (need - 1), tmp.end());
+                const uint64_t threshold = tmp[need - 1];
                 uint32_t span = steal_span(cfg);
-                for (size_t j = 0; j < need; ++j) {
-                    uint32_t i = order[j];
+                for (size_t i = 0; i < live; ++i) {
+                    if (keys[i] > threshold) continue;
                     uint32_t id = (uint32_t)pool.note_id[i];
                     pool.stop_rel[i] = id % span + 1;
                 }
@@ -557,23 +605,20 @@ struct Renderer {
         uint32_t region_shift = L[1], stage_shift = L[2], phase_shift = L[3], phase_mask = L[4],
                  dead_region = L[5];
         std::vector<uint32_t> key(n);
-        size_t live_new = 0;
-        for (size_t i = 0; i < n; ++i) {
-            uint32_t stage = pool.env_stage[i];
-            uint32_t region = dead_region;
-            if (stage != ENV_DEAD) {
-                region = std::min(pool.region[i], dead_region - 1);
-                ++live_new;
+        par_for(n, s.threads, [&](size_t lo, size_t hi) {
+            for (size_t i = lo; i < hi; ++i) {
+                uint32_t stage = pool.env_stage[i];
+                uint32_t region = dead_region;
+                if (stage != ENV_DEAD) region = std::min(pool.region[i], dead_region - 1);
+                uint32_t ph = ((uint32_t)(pool.phase[i] >> 32) >> phase_shift) & phase_mask;
+                key[i] = (region << region_shift) | (std::min(stage, 7u) << stage_shift) | ph;
             }
-            uint32_t ph = ((uint32_t)(pool.phase[i] >> 32) >> phase_shift) & phase_mask;
-            key[i] = (region << region_shift) | (std::min(stage, 7u) << stage_shift) | ph;
-        }
-        std::vector<uint32_t> perm(n);
-        std::iota(perm.begin(), perm.end(), 0u);
-        std::stable_sort(perm.begin(), perm.end(),
-                         [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
+        });
+        size_t live_new = 0;
+        for (size_t i = 0; i < n; ++i) live_new += pool.env_stage[i] != ENV_DEAD;
+        std::vector<uint32_t> perm = stable_order(key);
         perm.resize(live_new);
-        pool.permute(perm);
+        pool.permute(perm, s.threads);
     }
 };
 
@@ -607,62 +652,74 @@ int render_block(kor_synth *s, float *out32, double *out64, size_t n) {
 }  // namespace
 
 template <typename R>
-static void export_pool(const Pool<R> &p, uint32_t *o, size_t stride) {
-    size_t n = p.len();
-    for (size_t i = 0; i < n; ++i) {
-        o[0 * stride + i] = (uint32_t)p.phase[i];
-        o[1 * stride + i] = (uint32_t)(p.phase[i] >> 32);
-        o[2 * stride + i] = (uint32_t)p.step[i];
-        o[3 * stride + i] = (uint32_t)(p.step[i] >> 32);
-        o[4 * stride + i] = p.smp_base[i];
-        o[5 * stride + i] = p.smp_len[i];
-        o[6 * stride + i] = p.loop_start[i];
-        o[7 * stride + i] = p.loop_end[i];
-        o[8 * stride + i] = p.flags[i];
-        o[9 * stride + i] = p.env_stage[i];
-        o[10 * stride + i] = f32_bits((float)p.env_level[i]);
-        o[11 * stride + i] = f32_bits(p.gain_l[i]);
-        o[12 * stride + i] = f32_bits(p.gain_r[i]);
-        o[13 * stride + i] = f32_bits((float)p.filt_z1[i]);
-        o[14 * stride + i] = f32_bits((float)p.filt_z2[i]);
-        o[15 * stride + i] = p.params[i];
-        o[16 * stride + i] = p.region[i];
-        o[17 * stride + i] = p.gate_slot[i];
-        o[18 * stride + i] = p.ordinal[i];
-        o[19 * stride + i] = p.start_rel[i];
-        o[20 * stride + i] = (uint32_t)p.note_id[i];
-        o[21 * stride + i] = (uint32_t)(p.note_id[i] >> 32);
-        o[22 * stride + i] = p.born_variant[i];
-        o[23 * stride + i] = p.stop_rel[i];
-    }
+static void export_pool(const Pool<R> &p, uint32_t *o, size_t stride, int threads) {
+    const size_t n = p.len();
+    // One field per task, each written front to back.
+    auto field = [&](int f) {
+        uint32_t *d = o + (size_t)f * stride;
+        for (size_t i = 0; i < n; ++i) {
+            switch (f) {
+                case 0: d[i] = (uint32_t)p.phase[i]; break;
+                case 1: d[i] = (uint32_t)(p.phase[i] >> 32); break;
+                case 2: d[i] = (uint32_t)p.step[i]; break;
+                case 3: d[i] = (uint32_t)(p.step[i] >> 32); break;
+                case 4: d[i] = p.smp_base[i]; break;
+                case 5: d[i] = p.smp_len[i]; break;
+                case 6: d[i] = p.loop_start[i]; break;
+                case 7: d[i] = p.loop_end[i]; break;
+                case 8: d[i] = p.flags[i]; break;
+                case 9: d[i] = p.env_stage[i]; break;
+                case 10: d[i] = f32_bits((float)p.env_level[i]); break;
+                case 11: d[i] = f32_bits(p.gain_l[i]); break;
+                case 12: d[i] = f32_bits(p.gain_r[i]); break;
+                case 13: d[i] = f32_bits((float)p.filt_z1[i]); break;
+                case 14: d[i] = f32_bits((float)p.filt_z2[i]); break;
+                case 15: d[i] = p.params[i]; break;
+                case 16: d[i] = p.region[i]; break;
+                case 17: d[i] = p.gate_slot[i]; break;
+                case 18: d[i] = p.ordinal[i]; break;
+                case 19: d[i] = p.start_rel[i]; break;
+                case 20: d[i] = (uint32_t)p.note_id[i]; break;
+                case 21: d[i] = (uint32_t)(p.note_id[i] >> 32); break;
+                case 22: d[i] = p.born_variant[i]; break;
+                default: d[i] = p.stop_rel[i]; break;
+            }
+        }
+    };
+    par_for(24, threads > 1 && n >= 65536 ? 24 : 1, [&](size_t lo, size_t hi) {
+        for (size_t f = lo; f < hi; ++f) field((int)f);
+    });
 }
 
 template <typename R>
-static void import_pool(Pool<R> &p, const uint32_t *o, size_t n, size_t stride) {
+static void import_pool(Pool<R> &p, const uint32_t *o, size_t n, size_t stride, int threads) {
     p.for_each_field([&](auto &v) { v.resize(n); });
-    for (size_t i = 0; i < n; ++i) {
-        p.phase[i] = ((uint64_t)o[1 * stride + i] << 32) | o[0 * stride + i];
-        p.step[i] = ((uint64_t)o[3 * stride + i] << 32) | o[2 * stride + i];
-        p.smp_base[i] = o[4 * stride + i];
-        p.smp_len[i] = o[5 * stride + i];
-        p.loop_start[i] = o[6 * stride + i];
-        p.loop_end[i] = o[7 * stride + i];
-        p.flags[i] = o[8 * stride + i];
-        p.env_stage[i] = o[9 * stride + i];
-        p.env_level[i] = (R)bits_f32(o[10 * stride + i]);
-        p.gain_l[i] = bits_f32(o[11 * stride + i]);
-        p.gain_r[i] = bits_f32(o[12 * stride + i]);
-        p.filt_z1[i] = (R)bits_f32(o[13 * stride + i]);
-        p.filt_z2[i] = (R)bits_f32(o[14 * stride + i]);
-        p.params[i] = o[15 * stride + i];
-        p.region[i] = o[16 * stride + i];
-        p.gate_slot[i] = o[17 * stride + i];
-        p.ordinal[i] = o[18 * stride + i];
-        p.start_rel[i] = o[19 * stride + i];
-        p.note_id[i] = ((uint64_t)o[21 * stride + i] << 32) | o[20 * stride + i];
-        p.born_variant[i] = o[22 * stride + i];
-        p.stop_rel[i] = o[23 * stride + i];
-    }
+    auto w = [&](int f) { return o + (size_t)f * stride; };
+    par_for(n, threads, [&](size_t lo, size_t hi) {
+        for (size_t i = lo; i < hi; ++i) {
+            p.phase[i] = ((uint64_t)w(1)[i] << 32) | w(0)[i];
+            p.step[i] = ((uint64_t)w(3)[i] << 32) | w(2)[i];
+            p.smp_base[i] = w(4)[i];
+            p.smp_len[i] = w(5)[i];
+            p.loop_start[i] = w(6)[i];
+            p.loop_end[i] = w(7)[i];
+            p.flags[i] = w(8)[i];
+            p.env_stage[i] = w(9)[i];
+            p.env_level[i] = (R)bits_f32(w(10)[i]);
+            p.gain_l[i] = bits_f32(w(11)[i]);
+            p.gain_r[i] = bits_f32(w(12)[i]);
+            p.filt_z1[i] = (R)bits_f32(w(13)[i]);
+            p.filt_z2[i] = (R)bits_f32(w(14)[i]);
+            p.params[i] = w(15)[i];
+            p.region[i] = w(16)[i];
+            p.gate_slot[i] = w(17)[i];
+            p.ordinal[i] = w(18)[i];
+            p.start_rel[i] = w(19)[i];
+            p.note_id[i] = ((uint64_t)w(21)[i] << 32) | w(20)[i];
+            p.born_variant[i] = w(22)[i];
+            p.stop_rel[i] = w(23)[i];
+        }
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -929,16 +986,16 @@ size_t kor_pool_len(const kor_synth *s) {
 
 void kor_pool_export(const kor_synth *s, uint32_t *out, size_t stride) {
     if (s->precision)
-        export_pool(s->pool64, out, stride);
+        export_pool(s->pool64, out, stride, s->threads);
     else
-        export_pool(s->pool32, out, stride);
+        export_pool(s->pool32, out, stride, s->threads);
 }
 
 void kor_pool_import(kor_synth *s, const uint32_t *soa, size_t n, size_t stride) {
     if (s->precision)
-        import_pool(s->pool64, soa, n, stride);
+        import_pool(s->pool64, soa, n, stride, s->threads);
     else
-        import_pool(s->pool32, soa, n, stride);
+        import_pool(s->pool32, soa, n, stride, s->threads);
 }
 
 void kor_set_counters(kor_synth *s, uint64_t stolen, uint64_t dropped) {

@@ -219,6 +219,18 @@ def make_params_f64(attack_s, hold_s, decay_s, sustain, release_s, fc_cents, q_c
     return out[0]
 
 
+def design_table_f64(design: np.ndarray, cfg) -> np.ndarray:
+    """A bank's params table designed in f64 (`make_params_f64` per region), from
+    the design inputs the device's FP64 design takes (precision form)."""
+    out = np.zeros(len(design), PARAMS64_DTYPE)
+    for i, d in enumerate(design):
+        out[i] = make_params_f64(
+            d["attack_s"], d["hold_s"], d["decay_s"], d["sustain"], d["release_s"],
+            d["fc_cents"], d["q_cb"], float(cfg.sample_rate), cfg.decay_curve == 0,
+            cfg.release_curve == 0, float(np.float32(cfg.env_floor)), bool(d["filter"]))
+    return out
+
+
 # ---- the CPU backend --------------------------------------------------------
 
 class Oracle:

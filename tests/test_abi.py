@@ -62,3 +62,13 @@ def test_oracle_is_not_linked_into_the_product(lib):
         blob = f.read()
     assert b"kor_create" not in blob
     assert os.path.exists(oracle.LIB_PATH)
+
+
+def test_product_package_never_imports_the_oracle():
+    """The oracle is test infrastructure: nothing under kestrel_amd/ imports it."""
+    pkg = os.path.join(ROOT, "kestrel_amd")
+    pat = re.compile(r"^\s*(from\s+oracle\b|import\s+oracle\b|from\s+\.\.\s*oracle)", re.M)
+    for name in sorted(os.listdir(pkg)):
+        if name.endswith(".py"):
+            with open(os.path.join(pkg, name)) as f:
+                assert not pat.search(f.read()), name

@@ -103,7 +103,7 @@ def test_device_designed_stream_is_bit_exact_and_tracks_fp64():
     o32 = oracle.Oracle(cfg, bank.pool, table)
     o32.set_sorted(bank.n_regions, bank.max_sample_len, True)
     o64 = oracle.Oracle(cfg, bank.pool, table, precision=1)
-    o64.set_params64(bank.params_f64(cfg))
+    o64.set_params64(oracle.design_table_f64(bank.design, cfg))
     o64.set_sorted(bank.n_regions, bank.max_sample_len, True)
     bb = W.BlockBuilder(cfg, bank, notes, bends=W.random_walk_bends(17, 2.0, 0.05))
     err_g = err_c = peak = 0.0

@@ -461,3 +461,34 @@ def test_f64_render_tracks_f32_render(oracle_mod):
     e = np.concatenate(errs)
     assert 0 < np.max(np.abs(e)) < 5e-3
     assert np.sqrt(np.mean(e ** 2)) < 1e-3
+
+
+def test_sorted_mode_is_a_stable_sort_of_the_survivors(oracle_mod):
+    """Sorted mode (the order the device's locality sort leaves): the survivors
+    of a block, in their insertion order, stably sorted by the documented key
+    (region, stage, phase bucket; src/gpu/mod.rs:76-105). Many equal keys, so a
+    sort that is not stable, or a radix pass that drops a digit, fails."""
+    rng = np.random.default_rng(21)
+    n, regions = 3000, 5
+    cfg = small_cfg(max_voices=4096, block_frames=64, gate_frames=16)
+    c = spawn_many(oracle_mod, n, np.arange(n))
+    c["smp_len"] = rng.integers(5, 300, n)
+    c["region"] = rng.integers(0, regions, n)
+    c["phase_hi"] = rng.integers(0, 3, n)
+    c["step_hi"] = rng.integers(0, 3, n)
+    pools = []
+    for sort in (False, True):
+        o = oracle_mod.Oracle(cfg, np.ones(8192, np.int16), one_param(oracle_mod))
+        o.set_sorted(regions, 4096, sort)
+        o.set_gates(np.zeros(cfg.tiles() * 2048, np.uint32))
+        o.spawn(c)
+        o.render()
+        pools.append(o.pool_export())
+    plain, srt = pools
+    L = oracle_mod.sort_key_plan(regions, 4096)
+    key = ((np.minimum(plain[16], L["dead_region"] - 1) << L["region_shift"]) |
+           (np.minimum(plain[9], 7) << L["stage_shift"]) |
+           ((plain[1] >> L["phase_shift"]) & L["phase_mask"]))
+    order = np.argsort(key, kind="stable")
+    assert len(set(key.tolist())) < len(key) // 4  # plenty of ties
+    assert np.array_equal(srt, plain[:, order])
