@@ -52,9 +52,9 @@ def trace(d: str, blocks: int) -> str:
     rows.sort(key=lambda r: int(r["Start_Timestamp"]))
     # A block's chain opens with k_spawn (staged runs begin each block inside
     # the previous one's k_end); without spawns, with k_begin.
-    begins = [i for i, r in enumerate(rows) if r["Kernel_Name"].startswith("k_spawn")]
+    begins = [i for i, r in enumerate(rows) if short(r["Kernel_Name"]) == "k_spawn"]
     if len(begins) < blocks:
-        begins = [i for i, r in enumerate(rows) if r["Kernel_Name"].startswith("k_begin")]
+        begins = [i for i, r in enumerate(rows) if short(r["Kernel_Name"]) == "k_begin"]
     out = [f"### Last {blocks} blocks (from {os.path.basename(_one(d, 'kernel_trace.csv'))})", "",
            "| kernel | launches | mean us | share of block |", "|---|---:|---:|---:|"]
     if len(begins) >= blocks:
@@ -124,6 +124,9 @@ def main(argv=None):
         tr.add_argument("--" + k.replace("_", "-"), required=True)
     tr.add_argument("--kernel", default="k_render_fast")
     tr.add_argument("--last", type=int, default=10)
+    tr.add_argument("--workload", default=None, help="workload name recorded in the JSON")
+    tr.add_argument("--mean-voices", type=float, default=None,
+                    help="the profiled run's mean voices per block, recorded in the JSON")
     a = ap.parse_args(argv)
     if a.cmd == "trace":
         print(trace(a.dir, a.blocks))
@@ -150,9 +153,14 @@ def main(argv=None):
               f"| HBM bytes read per launch | {fetch_b:.4g} |\n"
               f"| HBM bytes written per launch | {write_b:.4g} |\n"
               f"| HBM bytes per launch | {fetch_b + write_b:.4g} |")
-        print(json.dumps({"kernel": a.kernel, "bytes_per_launch": round(fetch_b + write_b),
-                          "bytes_read": round(fetch_b), "bytes_written": round(write_b),
-                          "fetch_scale": round(kf, 5), "write_scale": round(kw, 5)}))
+        rec = {"kernel": a.kernel, "bytes_per_launch": round(fetch_b + write_b),
+               "bytes_read": round(fetch_b), "bytes_written": round(write_b),
+               "fetch_scale": round(kf, 5), "write_scale": round(kw, 5)}
+        if a.workload:
+            rec["workload"] = a.workload
+        if a.mean_voices:
+            rec["mean_voices"] = a.mean_voices
+        print(json.dumps(rec))
 
 
 if __name__ == "__main__":
