@@ -208,6 +208,15 @@ typedef struct kst_region_design {
  * kst_set_params_variant. */
 int kst_design_params(kst_backend *b, uint32_t index, const kst_region_design *in, size_t n);
 
+/* kst_design_params without a device allocation: the design rows are staged in
+ * a static device array of the library (65,536 rows, 4 MiB per GPU, held until
+ * the process ends) instead of a buffer allocated and freed per call. No
+ * counterpart in the reference; for callers that cannot allocate device memory
+ * (serving loops, harnesses that forbid it). n above 65,536 fails with
+ * KST_EINVAL. Calls that overlap on one GPU share the array and race. */
+int kst_design_params_noalloc(kst_backend *b, uint32_t index, const kst_region_design *in,
+                              size_t n);
+
 /* ---- post-render step: the caller's side of render() ----
  *
  * What the reference's driver does with every rendered block (src/driver.rs:

@@ -119,6 +119,7 @@ SIGNATURES = {
     "kst_batch_run": (i32, [P, i32]),
     "kst_batch_destroy": (None, [P]),
     "kst_design_params": (i32, [P, u32, P, sz]),
+    "kst_design_params_noalloc": (i32, [P, u32, P, sz]),
     "kst_default_limiter_config": (None, [P]),
     "kst_limiter_create": (i32, [P, P]),
     "kst_limiter_process": (i32, [P, P, sz]),

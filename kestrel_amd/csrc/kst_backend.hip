@@ -773,6 +773,35 @@ int kst_design_params(kst_backend *b, uint32_t index, const kst_region_design *i
     return KST_OK;
 }
 
+int kst_design_params_noalloc(kst_backend *b, uint32_t index, const kst_region_design *in,
+                              size_t n) {
+    if (b && b->inflight) return fail(KST_ESTATE, "a submitted render has not been collected");
+    if (!b || (!in && n)) return fail(KST_EINVAL, "null argument");
+    if (n != b->ppv)
+        return fail(KST_EINVAL, "params variant %u has %zu entries, expected %u", index, n, b->ppv);
+    if (index >= b->cfg.max_param_variants)
+        return fail(KST_EINVAL, "params variant %u is past the configured maximum", index);
+    if (n > kst::DESIGN_SCRATCH_ROWS)
+        return fail(KST_EINVAL, "%zu design rows exceed the allocation-free entry point's "
+                                "%zu-row scratch", n, kst::DESIGN_SCRATCH_ROWS);
+    HIPCHK(hipSetDevice(b->device));
+    kst_region_design *d = nullptr;
+    hipError_t e = kst::design_scratch(b->device, &d);
+    if (e != hipSuccess) return fail(KST_EDEVICE, "design scratch address: %s", hipGetErrorString(e));
+    // Every row the kernel reads is written here first, so nothing of an
+    // earlier call's rows is read.
+    e = hipMemcpyAsync(d, in, n * sizeof(kst_region_design), hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) {
+        launch_design(d, (uint32_t)n, b->params.p + (size_t)index * b->ppv,
+                      (double)b->cfg.sample_rate, b->cfg.decay_curve == 0,
+                      b->cfg.release_curve == 0, b->cfg.env_floor, b->stream);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) return fail(KST_EDEVICE, "designing params: %s", hipGetErrorString(e));
+    return KST_OK;
+}
+
 int kst_params_export(kst_backend *b, uint32_t index, kst_region_params *out, size_t n) {
     if (!b || (!out && n)) return fail(KST_EINVAL, "null argument");
     if (n != b->ppv)

@@ -18,7 +18,38 @@
 // the resonance gain is the power 2^(-1/4 - e/2) of the same exponent e.
 #include "kst_device.h"
 
+#include <mutex>
+
 namespace kst {
+
+// Staging for the allocation-free design entry point: the caller's design rows
+// are copied here instead of into a buffer allocated per call. One copy per
+// device (every GPU the library runs on holds it until the process ends); host
+// code uses the address hipGetSymbolAddress returns for the current device,
+// never the array's name. Calls that overlap on one device share it and race.
+static __device__ __align__(256) kst_region_design scratch[DESIGN_SCRATCH_ROWS];
+
+namespace {
+std::mutex scratch_mu;
+constexpr int SCRATCH_DEVICES = 64;
+kst_region_design *scratch_ptr[SCRATCH_DEVICES] = {};
+}  // namespace
+
+// The scratch array's device address on `device`, which must be current.
+hipError_t design_scratch(int device, kst_region_design **out) {
+    *out = nullptr;
+    if (device < 0 || device >= SCRATCH_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        scratch_ptr[device] = static_cast<kst_region_design *>(p);
+    }
+    *out = scratch_ptr[device];
+    return hipSuccess;
+}
+
 namespace {
 
 constexpr double LOG2_10 = 3.321928094887362;

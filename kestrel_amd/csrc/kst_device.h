@@ -242,6 +242,9 @@ void launch_batch_reduce(const Args *aa, uint32_t nr, bool comp, uint32_t max_tw
 void launch_batch_sort(const Args *aa, uint32_t nr, uint32_t grid, uint32_t gather_grid,
                        uint32_t passes, uint32_t max_radix, hipStream_t s);
 void launch_batch_end(const Args *aa, uint32_t nr, hipStream_t s);
+// Rows of the allocation-free design entry point's device scratch.
+constexpr size_t DESIGN_SCRATCH_ROWS = 65536;
+hipError_t design_scratch(int device, kst_region_design **out);
 void launch_design(const kst_region_design *in, uint32_t n, kst_region_params *out, double sr,
                    bool exp_decay, bool exp_release, float env_floor, hipStream_t s);
 void launch_reduce_fast(const Args &a, bool comp, uint32_t max_rows, hipStream_t s);
