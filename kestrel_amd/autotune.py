@@ -1,12 +1,17 @@
-"""Autotuner for the batching form: block length and waves per render
-workgroup, per class of render (pool size by powers of two, interpolation).
+"""Autotuner for the batching form: block length, waves per render workgroup
+and the locality sort's radix width, per class of render (pool size by powers
+of two, interpolation).
 
 Each candidate is timed on the device on a short synthetic run of its class
 (a pool filled by bursts of note-ons, then a few staged blocks), and the one
-with the most voice-samples per second wins. Block length changes the output
-(gate cadence, steal span), so a render's block length is fixed by the tuner
-before its input stream is built; waves per workgroup and the sort radix only
-change timing.
+with the most voice-samples per second wins. The search is coordinate-wise:
+block length over 256-8192 frames (the reference's default is 4096) with one
+wave per workgroup and 9-bit digits, then waves per workgroup at the chosen
+length, then the radix width at both. Block length changes the output (gate
+cadence, steal span), so a render's block length is fixed by the tuner before
+its input stream is built; waves per workgroup and the radix width only change
+timing. Radix widths stop at 9 bits: the sort's scatter keeps per-digit
+tables for every wave of its workgroup in LDS.
 """
 from __future__ import annotations
 
@@ -18,7 +23,9 @@ import numpy as np
 from . import workload as W
 from .config import Config
 
-CANDIDATES = [(f, w) for f in (256, 512, 1024) for w in (1, 2)]
+BLOCK_FRAMES = (256, 512, 1024, 2048, 4096, 8192)
+WAVES = (1, 2)
+DIGIT_BITS = (6, 8, 9)
 MIXED_WPG_MARGIN = 1.03
 
 
@@ -35,11 +42,16 @@ def _stage(g, blocks):
 
 
 def measure(max_voices: int, interpolation: int, block_frames: int, waves_per_group: int,
-            warm_s: float = 0.25, timed_s: float = 0.1, seed: int = 0x4B53A7) -> float:
-    """Voice-samples/s of one candidate on a saturated synthetic pool."""
+            digit_bits: int = 9, warm_s: float = 0.25, timed_s: float = 0.1,
+            seed: int = 0x4B53A7) -> float:
+    """Voice-samples/s of one candidate on a saturated synthetic pool (at least
+    four timed blocks)."""
     from .backend import HipSynth
     cfg = Config(block_frames=block_frames, max_voices=max_voices, interpolation=interpolation,
-                 max_steal_percent=10, compensated=True, waves_per_group=waves_per_group)
+                 max_steal_percent=10, compensated=True, waves_per_group=waves_per_group,
+                 sort_digit_bits=digit_bits)
+    n_timed = max(4, int(timed_s * 48000 / block_frames))
+    timed_s = n_timed * block_frames / 48000
     bank = W.make_bank(seed, instruments=8, zones=4, key_lo=0, key_hi=127, frames=16384,
                        loop=(4096, 15000))
     seconds = warm_s + timed_s + 0.05
@@ -54,7 +66,7 @@ def measure(max_voices: int, interpolation: int, block_frames: int, waves_per_gr
         _stage(g, warm)
         g.run_staged(True)
         live0 = g.stats()["active_voices"]
-        timed = [bb.next_block() for _ in range(max(1, int(timed_s * 48000 / block_frames)))]
+        timed = [bb.next_block() for _ in range(n_timed)]
         _stage(g, timed)
         t0 = time.perf_counter()
         g.run_staged(True)
@@ -70,19 +82,27 @@ def measure(max_voices: int, interpolation: int, block_frames: int, waves_per_gr
 
 
 def tune(specs: list[dict], log=print) -> dict:
-    """{(size class, interpolation): (block_frames, waves_per_group)} for the
-    classes the specs use."""
+    """{(size class, interpolation): (block_frames, waves_per_group, digit_bits)}
+    for the classes the specs use."""
     classes = sorted({(size_class(s["max_voices"]), s["interpolation"]) for s in specs})
     best = {}
     for cls, interp in classes:
         pool = 1 << cls
-        scores = {c: measure(pool, interp, *c) for c in CANDIDATES}
+        scores = {}
+
+        def score(c):
+            if c not in scores:
+                scores[c] = measure(pool, interp, *c)
+            return scores[c]
+
+        f = max(BLOCK_FRAMES, key=lambda f: score((f, 1, 9)))
         # In a batch, members that differ in waves per workgroup need render
         # launches of their own, run one after the other; two waves per
         # workgroup has to win clearly to be worth that.
-        eff = {(f, w): v / (MIXED_WPG_MARGIN if w != 1 else 1.0) for (f, w), v in scores.items()}
-        pick = max(eff, key=eff.get)
-        best[(cls, interp)] = pick
+        w = max(WAVES, key=lambda w: score((f, w, 9)) / (MIXED_WPG_MARGIN if w != 1 else 1.0))
+        b = max(DIGIT_BITS, key=lambda b: score((f, w, b)))
+        best[(cls, interp)] = (f, w, b)
         log(f"autotune pool 2^{cls} interp {interp}: " +
-            ", ".join(f"{f}/{w}: {v:.2e}" for (f, w), v in scores.items()) + f" -> {pick}")
+            ", ".join(f"{c[0]}/{c[1]}/{c[2]}: {v:.2e}" for c, v in scores.items()) +
+            f" -> {(f, w, b)}")
     return best

@@ -717,12 +717,13 @@ def c4_specs(seed: int = 0x4B530004, n: int = 64) -> list[dict]:
     return out
 
 
-def c4_render(spec: dict, window_s: float, block_frames: int = 512, waves_per_group: int = 1):
+def c4_render(spec: dict, window_s: float, block_frames: int = 512, waves_per_group: int = 1,
+              sort_digit_bits: int = 9):
     """(cfg, bank, notes) for one batch member: its first `window_s` seconds of
     Poisson note-ons at its own rate, exponential durations (mean 1 s)."""
     cfg = Config(block_frames=block_frames, max_voices=spec["max_voices"],
                  interpolation=spec["interpolation"], max_steal_percent=10, compensated=True,
-                 waves_per_group=waves_per_group)
+                 waves_per_group=waves_per_group, sort_digit_bits=sort_digit_bits)
     frames = spec["frames"]
     bank = make_bank(spec["seed"], instruments=spec["instruments"], zones=4, key_lo=0,
                      key_hi=127, frames=frames, loop=(frames // 4, frames - frames // 16))
