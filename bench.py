@@ -453,7 +453,23 @@ def run_c5(args):
     x_rt = K * F / cfg.sample_rate / wall
     log(f"timed: {K} blocks in {wall * 1e3:.1f} ms, mean voices {V.mean():.0f}, {value:.3e} "
         f"voice-samples/s, {x_rt:.3f}x realtime; per-pass ms/block {timings}")
+    # The render kernel's algorithmic bytes per launch (cubic: four taps).
+    step = (snapshot[3].astype(np.float64) * 2 ** 32 + snapshot[2]) / 2 ** 32
+    mean_step = float(step.mean()) if step.size else 1.0
+    spawned = np.array([min(len(b.cmds), cfg.max_voices) for b in timed], np.float64)
+    bytes_block = V * (192.0 + 2.0 * 3 + 2.0 * F * mean_step) + 76.0 * spawned + 8.0 * F
+    roofline = None
+    if timings.get("render"):
+        achieved = float(bytes_block.mean()) / (timings["render"] * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                    "kernel": "k_render_fast",
+                    "algorithmic_bytes_per_launch": round(float(bytes_block.mean())),
+                    "bytes_per_voice_sample": round(float(bytes_block.sum() / vs), 4),
+                    "render_ms_per_block": round(timings["render"], 4)}
+    log(f"roofline: {roofline}")
     parity = {}
+    cpu_baseline = None
     if not args.no_cpu:
         from oracle import oracle
         gpu.pool_import(snapshot)
@@ -484,6 +500,36 @@ def run_c5(args):
                       ref.shape == after.shape and np.array_equal(ref[keep], after[keep]))}
         log(f"cpu parity: {parity}")
         del o
+        # Throughput sample of the full CPU render: the oracle over a slice of
+        # the snapshot (its first 2,097,152 voices), one block with no note-ons.
+        slice_n = min(1 << 21, snapshot.shape[1])
+        zero_cmds = np.zeros(0, timed[0].cmds.dtype)
+        tc_all, tc_one = None, None
+        for threads in (host_threads(), 1):
+            o = oracle.Oracle(cfg, bank.pool, bank.params)
+            o.set_sorted(bank.n_regions, bank.max_sample_len, True)
+            o.set_threads(threads)
+            o.pool_import(np.ascontiguousarray(snapshot[:, :slice_n]))
+            o.set_gates(blk.gates)
+            o.set_channels(blk.chans, *blk.flags)
+            o.spawn(zero_cmds)
+            tc = time.perf_counter()
+            o.render_f64()
+            tc = time.perf_counter() - tc
+            if threads == 1:
+                tc_one = tc
+            else:
+                tc_all = tc
+            del o
+        sample = f"one block of {F} frames over the first {slice_n} voices of the pool snapshot"
+        cpu_baseline = {"value": round(slice_n * F / tc_all, 1), "unit": "voice-samples/s",
+                        "cores": host_threads(), "kind": "port",
+                        "sample": sample + f" ({tc_all:.1f} s), every host thread",
+                        "cores_1": {"value": round(slice_n * F / tc_one, 1),
+                                    "unit": "voice-samples/s", "threads": 1,
+                                    "sample": sample + f" ({tc_one:.1f} s)"},
+                        "host_cores": os.cpu_count(), "cpu_model": cpu_model()}
+        log(f"cpu: {cpu_baseline}")
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "voice-samples/s", "n_gpus": 1,
         "steps": K, "warmup": WU, "ms_per_step": round(wall / K * 1e3, 4),
@@ -492,7 +538,8 @@ def run_c5(args):
         "config": {"workload": WORKLOADS["c5"], "name": "c5", "block_frames": F,
                    "max_voices": cfg.max_voices, "mean_voices": round(float(V.mean()), 1),
                    "form": gpu.name(), "graph": bool(args.graph)},
-        "x_realtime": round(x_rt, 4), "timings_ms": timings, "parity": parity,
+        "x_realtime": round(x_rt, 4), "timings_ms": timings, "roofline": roofline,
+        "cpu_baseline": cpu_baseline, "parity": parity,
     }
     gpu.close()
     print(json.dumps(line), flush=True)
