@@ -24,6 +24,14 @@
 // bitwise identical.
 #include "kst_device.h"
 
+// The structured buffer load (LLVM's llvm.amdgcn.struct.buffer.load.i32, which
+// this clang has no builtin for): the 4-byte word at descriptor base + vindex *
+// stride + voffset + soffset, with idxen addressing. Declared at file scope so
+// the name binds to the intrinsic.
+typedef int kst_rsrc_t __attribute__((ext_vector_type(4)));
+__device__ int kst_struct_load_i32(kst_rsrc_t rsrc, int vindex, int voffset, int soffset, int aux)
+    __asm("llvm.amdgcn.struct.buffer.load.i32");
+
 namespace kst {
 namespace {
 
@@ -435,6 +443,24 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t bank_rsrc(const Args &a) {
                                              0x00020000);
 }
 
+// The bank as a structured buffer of 2-byte records: a tap load's index is the
+// frame number itself, so a phase chain that carries the sample's base in its
+// high word gives each tap's address with no arithmetic of its own. Dword 1
+// holds the address's high bits and the stride (bits 16-29); dword 2 the record
+// count (the pool and its guard frames). readfirstlane keeps the descriptor in
+// scalar registers (a descriptor the compiler places in vector registers turns
+// every load into a loop over lanes).
+__device__ __forceinline__ kst_rsrc_t bank_records(const Args &a) {
+    const uint64_t nr = (uint64_t)a.bank_len + 8;
+    const uint64_t ba = (uint64_t)a.bank;
+    kst_rsrc_t r;
+    r.x = __builtin_amdgcn_readfirstlane((int)(uint32_t)ba);
+    r.y = __builtin_amdgcn_readfirstlane((int)(((uint32_t)(ba >> 32) & 0xFFFFu) | (2u << 16)));
+    r.z = __builtin_amdgcn_readfirstlane((int)(nr > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)nr));
+    r.w = 0x00020000;
+    return r;
+}
+
 // One 32-bit word holding the int16 frames at byte offsets off and off + 2
 // (gfx950 buffer loads take 2-byte-aligned offsets).
 __device__ __forceinline__ uint32_t tap_pair(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
@@ -488,7 +514,24 @@ __device__ __forceinline__ void neutralise(Voice &v) {
 template <int INTERP>
 __device__ __forceinline__ void straight_taps(const Args &a, const Voice &v,
                                               uint32_t (&w0)[SUBTILE],
-                                              uint32_t (&w1)[INTERP == 2 ? SUBTILE : 1]) {
+                                              uint32_t (&w1)[INTERP == 2 ? SUBTILE : 1],
+                                              uint32_t (&lo)[SUBTILE]) {
+    if (INTERP != 2) {
+        // Nearest and linear: the chain runs phase + (base << 32), whose high
+        // word is the tap's frame in the pool and whose low word is the
+        // phase's own (lo, for the frame loop's fraction).
+        const kst_rsrc_t bank = bank_records(a);
+        uint64_t pb = v.phase + ((uint64_t)v.base << 32);
+#pragma unroll
+        for (uint32_t i = 0; i < SUBTILE; ++i) {
+            w0[i] = (uint32_t)kst_struct_load_i32(bank, (int)(uint32_t)(pb >> 32), 0, 0, 0);
+            lo[i] = (uint32_t)pb;
+            pb += v.step;
+        }
+        return;
+    }
+    // Cubic keeps its 24-bit offset arithmetic: twice the taps in flight and
+    // a 32-bit fraction chain leave it no registers for a second phase chain.
     const __amdgpu_buffer_rsrc_t bank = bank_rsrc(a);
     // Byte offset 2 * (base + idx) as one 24-bit multiply-add of the phase's
     // high word (idx < 2^24 on the straight path, which also keeps base + idx
@@ -553,13 +596,14 @@ __device__ __forceinline__ uint64_t wrap_taps(const Args &a, const Voice &v,
 }
 
 // The straight path's frame loop over taps already loaded (neutralised lanes).
+// lo: nearest and linear take each frame's phase low word from the tap chain.
 // WRAP: the phase after the sub-tile is `wrapped_end` (see wrap_taps).
 template <int INTERP, bool CLAMP, int FILT, bool ENV_STATIC, bool FADE, bool WRAP = false>
 __device__ __forceinline__ void fast_subtile(const Args &a, Voice &v, float *tb,
                                              const float *ftab, uint32_t lane, uint32_t f0,
                                              const uint32_t (&w0)[SUBTILE],
                                              const uint32_t (&w1)[INTERP == 2 ? SUBTILE : 1],
-                                             uint64_t wrapped_end = 0) {
+                                             const uint32_t *lo, uint64_t wrapped_end = 0) {
     float m, c, thi, tlo;
     env_coefs(a, v, m, c, thi, tlo);
     const bool filt = FILT == 1 || (FILT == 2 && (v.p.flags & RP_FILTER));
@@ -569,8 +613,8 @@ __device__ __forceinline__ void fast_subtile(const Args &a, Voice &v, float *tb,
     int32_t d = (int32_t)(f0 + 1) - (int32_t)v.stop_rel;
     const int32_t fade = (int32_t)a.fade;
     bool died = false;
-    // Cubic keeps twice the taps in flight: its frame loop runs a 32-bit
-    // chain of the phase's low word instead of sharing the tap loop's.
+    // Cubic (and the wrap variant) runs its own 32-bit chain of the phase's
+    // low word.
     uint32_t lo32 = (uint32_t)v.phase;
 #pragma unroll
     for (uint32_t i = 0; i < SUBTILE; ++i) {
@@ -591,7 +635,7 @@ __device__ __forceinline__ void fast_subtile(const Args &a, Voice &v, float *tb,
             }
             g = fminf(v.level, 1.0f);
         }
-        const float flo = (float)(INTERP == 2 || WRAP ? lo32 : (uint32_t)v.phase);
+        const float flo = (float)(INTERP == 2 || WRAP ? lo32 : lo[i]);
         float s;
         if (INTERP == 0) {
             s = norm_lo<CLAMP>(w0[i]);
@@ -635,12 +679,10 @@ __device__ __forceinline__ void fast_subtile(const Args &a, Voice &v, float *tb,
         }
         if (INTERP == 2 || WRAP)
             lo32 += (uint32_t)v.step;
-        else
-            v.phase += v.step;
     }
     if (WRAP)
         v.phase = wrapped_end;
-    else if (INTERP == 2)
+    else
         v.phase += v.step * SUBTILE;  // the same sum, mod 2^64
     if (!ENV_STATIC && kill && v.stage == ENV_DEAD) v.gl = v.gr = 0.0f;
     if (FADE && died) {
@@ -660,17 +702,17 @@ __device__ __forceinline__ void fast_subtile_dispatch(const Args &a, Voice &v, f
     const bool all_filtered = !__any(sounding && !f);
     const bool none_filtered = !__any(sounding && f);
     neutralise<INTERP>(v);
-    uint32_t w0[SUBTILE], w1[INTERP == 2 ? SUBTILE : 1];
-    straight_taps<INTERP>(a, v, w0, w1);
+    uint32_t w0[SUBTILE], w1[INTERP == 2 ? SUBTILE : 1], lo[SUBTILE];
+    straight_taps<INTERP>(a, v, w0, w1, lo);
     if (all_filtered) {
         if (env_static)
-            fast_subtile<INTERP, CLAMP, 1, true, false>(a, v, tb, ftab, lane, f0, w0, w1);
+            fast_subtile<INTERP, CLAMP, 1, true, false>(a, v, tb, ftab, lane, f0, w0, w1, lo);
         else
-            fast_subtile<INTERP, CLAMP, 1, false, FADE>(a, v, tb, ftab, lane, f0, w0, w1);
+            fast_subtile<INTERP, CLAMP, 1, false, FADE>(a, v, tb, ftab, lane, f0, w0, w1, lo);
     } else if (none_filtered) {
-        fast_subtile<INTERP, CLAMP, 0, false, FADE>(a, v, tb, ftab, lane, f0, w0, w1);
+        fast_subtile<INTERP, CLAMP, 0, false, FADE>(a, v, tb, ftab, lane, f0, w0, w1, lo);
     } else {
-        fast_subtile<INTERP, CLAMP, 2, false, FADE>(a, v, tb, ftab, lane, f0, w0, w1);
+        fast_subtile<INTERP, CLAMP, 2, false, FADE>(a, v, tb, ftab, lane, f0, w0, w1, lo);
     }
 }
 
@@ -906,7 +948,7 @@ __device__ __forceinline__ void render_fast_body(const Args &a) {
             neutralise<INTERP>(v);
             uint32_t w0[SUBTILE], w1[INTERP == 2 ? SUBTILE : 1];
             const uint64_t end = wrap_taps<INTERP>(a, v, w0, w1);
-            fast_subtile<INTERP, CLAMP, 2, false, false, true>(a, v, tb, ftab, lane, f0, w0, w1, end);
+            fast_subtile<INTERP, CLAMP, 2, false, false, true>(a, v, tb, ftab, lane, f0, w0, w1, nullptr, end);
         } else if (INTERP != 2 && !__any(!edge_ok<INTERP>(a, v, f0))) {
             fast_subtile_edge<INTERP, CLAMP>(a, v, tb, ftab, lane, f0);
         } else {
