@@ -516,39 +516,20 @@ __device__ __forceinline__ void straight_taps(const Args &a, const Voice &v,
                                               uint32_t (&w0)[SUBTILE],
                                               uint32_t (&w1)[INTERP == 2 ? SUBTILE : 1],
                                               uint32_t (&lo)[SUBTILE]) {
-    if (INTERP != 2) {
-        // Nearest and linear: the chain runs phase + (base << 32), whose high
-        // word is the tap's frame in the pool and whose low word is the
-        // phase's own (lo, for the frame loop's fraction).
-        const kst_rsrc_t bank = bank_records(a);
-        uint64_t pb = v.phase + ((uint64_t)v.base << 32);
-#pragma unroll
-        for (uint32_t i = 0; i < SUBTILE; ++i) {
-            w0[i] = (uint32_t)kst_struct_load_i32(bank, (int)(uint32_t)(pb >> 32), 0, 0, 0);
-            lo[i] = (uint32_t)pb;
-            pb += v.step;
-        }
-        return;
-    }
-    // Cubic keeps its 24-bit offset arithmetic: twice the taps in flight and
-    // a 32-bit fraction chain leave it no registers for a second phase chain.
-    const __amdgpu_buffer_rsrc_t bank = bank_rsrc(a);
-    // Byte offset 2 * (base + idx) as one 24-bit multiply-add of the phase's
-    // high word (idx < 2^24 on the straight path, which also keeps base + idx
-    // + reach in the bank). The phase chain is the one the frame loop runs, so
-    // the compiler shares it.
-    const uint32_t base2 = v.base * 2u - (INTERP == 2 ? 2u : 0u);
-    uint64_t ph = v.phase;
+    // The chain runs phase + (base << 32) (cubic: base - 1, its first tap
+    // being idx - 1), whose high word is the tap's frame in the pool and whose
+    // low word is the phase's own (lo, the frame loop's fraction for nearest
+    // and linear). A cubic frame's second word, frames idx + 1 and idx + 2, is
+    // 4 bytes on: the instruction's own offset.
+    const kst_rsrc_t bank = bank_records(a);
+    uint64_t pb = v.phase + ((uint64_t)(v.base - (INTERP == 2 ? 1u : 0u)) << 32);
 #pragma unroll
     for (uint32_t i = 0; i < SUBTILE; ++i) {
-        const uint32_t off = __umul24((uint32_t)(ph >> 32), a.tap_bytes) + base2;
-        if (INTERP == 2) {
-            w0[i] = tap_pair(bank, off);      // frames idx-1, idx
-            w1[i] = tap_pair(bank, off + 4);  // frames idx+1, idx+2
-        } else {
-            w0[i] = tap_pair(bank, off);      // frames idx, idx+1
-        }
-        ph += v.step;
+        const int r = (int)(uint32_t)(pb >> 32);
+        w0[i] = (uint32_t)kst_struct_load_i32(bank, r, 0, 0, 0);
+        if (INTERP == 2) w1[i] = (uint32_t)kst_struct_load_i32(bank, r, 4, 0, 0);
+        lo[i] = (uint32_t)pb;
+        pb += v.step;
     }
 }
 
