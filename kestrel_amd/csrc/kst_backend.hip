@@ -187,6 +187,10 @@ struct kst_backend {
     DevBuf<float> partials, rsum, fade_tab;
     uint32_t fade_lds = 0;
     DevBuf<uint32_t> keys[2], idx[2], sort_hist, sort_tot, sel_cnt, cand_slot;
+    // The per-block render's dispatch lists (wave64 form): per-batch victim
+    // flags and one list per XCD of order_stride entries.
+    DevBuf<uint32_t> batch_victim, render_order;
+    uint32_t order_stride = 0;
     DevBuf<unsigned long long> cand_key, sel_min, sel_max;
     DevBuf<DevState> st;
 
@@ -404,6 +408,9 @@ struct kst_backend {
         a.sel_max = sel_max.p;
         a.cand_key = cand_key.p;
         a.cand_slot = cand_slot.p;
+        a.batch_victim = batch_victim.p;
+        a.render_order = render_order.p;
+        a.order_stride = order_stride;
         return a;
     }
 
@@ -435,7 +442,10 @@ struct kst_backend {
         // block is begun before its loop).
         if (!pipelined) launch_begin(a, s);
         if (timed) (void)hipEventRecord(ev[1], s);
-        if (spawn) launch_spawn(a, per_voice_grid(n_total), s);
+        // The wave64 form's k_spawn also flags the batches holding a victim,
+        // from which k_order builds the render's dispatch lists.
+        if (spawn || fast) launch_spawn(a, per_voice_grid(n_total), s);
+        if (fast) launch_order(a, s);
         if (timed) (void)hipEventRecord(ev[2], s);
         if (fast)
             launch_render_fast(a, chan, s);
@@ -674,6 +684,12 @@ int kst_create(const kst_config *cfg, const int16_t *pool, size_t pool_len,
         b->cand_key.alloc(b->slots) != hipSuccess || b->cand_slot.alloc(b->slots) != hipSuccess ||
         b->st.alloc(1) != hipSuccess)
         return cleanup(fail(KST_EDEVICE, "allocating the select/sort scratch"));
+    if (b->fast) {
+        b->order_stride = (b->max_rows + XCDS - 1) / XCDS + 2;
+        if (b->batch_victim.alloc(b->max_rows + 1) != hipSuccess ||
+            b->render_order.alloc((size_t)XCDS * b->order_stride) != hipSuccess)
+            return cleanup(fail(KST_EDEVICE, "allocating the render's dispatch lists"));
+    }
     // The steal fade's gain for d = 0 .. fade-1 (src/cpu.rs:577-586): the same
     // correctly rounded f32 division and subtraction the per-frame path does.
     {
@@ -1152,6 +1168,8 @@ int kst_batch_run(kst_batch *bt, int use_graph) {
         const uint32_t n = b->slots;
         host[i] = b->args(b->render_groups_for(n));
         host[i].fused = 0;  // a batch compacts with its own gather pass
+        host[i].batch_victim = nullptr;  // and maps its render workgroups statically
+        host[i].render_order = nullptr;
         steps = std::max(steps, b->staged);
         sel = std::max(sel, std::max(1u, std::min(std::min(b->cus, SEL_MAX_BLOCKS),
                                                   (b->cfg.max_voices + SEL_THREADS * 4 - 1) /

@@ -71,6 +71,9 @@ constexpr uint32_t FADE_LDS_MAX = 2048;
 constexpr uint32_t TBUF_FLOATS = 2 * SUBTILE * TROW;  // 32 rows
 // Batch rows summed per workgroup in the first mixdown stage.
 constexpr uint32_t REDUCE_ROWS = 256;
+// Workgroups are dealt to the XCDs in turn (workgroup w runs on XCD w mod 8,
+// up to a rotation), and each XCD starts its workgroups in order.
+constexpr uint32_t XCDS = 8;
 
 // Device-resident state of one backend, carried across blocks. The live count
 // lives here rather than in a kernel argument so a whole block of passes, or a
@@ -81,7 +84,6 @@ constexpr uint32_t REDUCE_ROWS = 256;
 // L2 holds the samples of a few regions rather than of all of them. A
 // bijection on [0, n): XCD x takes groups [x*q + min(x, r), ... + q + (x < r)).
 __device__ __forceinline__ uint32_t xcd_group(uint32_t id, uint32_t n) {
-    constexpr uint32_t XCDS = 8;
     const uint32_t x = id % XCDS, k = id / XCDS;
     const uint32_t q = n / XCDS, r = n % XCDS;
     return x * q + (x < r ? x : r) + k;
@@ -92,7 +94,6 @@ __device__ __forceinline__ uint32_t xcd_group(uint32_t id, uint32_t n) {
 // new voices (late starts, the slowest waves), so they start first instead of
 // trailing the launch.
 __device__ __forceinline__ uint32_t xcd_group_rev(uint32_t id, uint32_t n) {
-    constexpr uint32_t XCDS = 8;
     const uint32_t x = id % XCDS, k = id / XCDS;
     const uint32_t q = n / XCDS, r = n % XCDS;
     const uint32_t cnt = q + (x < r ? 1u : 0u);
@@ -193,6 +194,14 @@ struct Args {
     unsigned long long *sel_max;   // [SEL_MAX_BLOCKS][SEL_BINS]
     unsigned long long *cand_key;  // [cap]
     uint32_t *cand_slot;           // [cap]
+    // Per-block render dispatch order (wave64, fused; null: the plain XCD
+    // runs). batch_victim[j]: survivor batch j holds a steal victim (written
+    // by k_spawn). render_order[x * order_stride + k]: the k-th batch XCD x
+    // renders (written by k_order): its share of the new voices' batches, its
+    // run's batches holding a victim, then the run's other batches.
+    uint32_t *batch_victim;  // [cap / 64 + 1]
+    uint32_t *render_order;  // [XCDS][order_stride]
+    uint32_t order_stride;
 };
 
 // One of a pair of buffers, chosen without indexing the kernel-argument array
@@ -214,6 +223,8 @@ void launch_select(const Args &a, uint32_t grid, uint32_t rounds, hipStream_t s)
 void launch_spawn(const Args &a, uint32_t grid, hipStream_t s);
 void launch_sort(const Args &a, uint32_t grid, uint32_t gather_grid, hipStream_t s);
 void launch_end(const Args &a, hipStream_t s);
+// The per-block render's dispatch lists from k_spawn's victim flags.
+void launch_order(const Args &a, hipStream_t s);
 // The per-block inputs of a single submitted block (channel flags, command
 // count, command offset 0, block index 0), written from kernel arguments.
 void launch_stage_one(const Args &a, uint32_t chan_flags, uint32_t ncmd, hipStream_t s);

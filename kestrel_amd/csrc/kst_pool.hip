@@ -651,8 +651,25 @@ __global__ void __launch_bounds__(SEL_THREADS) kb_sel_final(const Args *__restri
 
 // shaders/spawn.wgsl:35-78; thinning per src/voice.rs:72-75 when the block is
 // oversubscribed under AdmitRule::Even (src/gpu/mod.rs:1056-1071).
+// Which batches of the live pool hold a steal victim, for the render's
+// dispatch order: batch j (voices 64j .. 64j + 63, all survivors) when some
+// voice's stop_rel is set, read where the render reads it (slot perm[vi] of
+// the live pool). One wave per batch, grid-stride.
+__device__ __forceinline__ void victim_flags(const Args &a, const DevState *st) {
+    const uint32_t nsurv = st->live / 64;
+    const uint32_t *pool = sel2(a.pools, st->parity);
+    const uint32_t *perm = live_perm(a);
+    const uint32_t stride = gridDim.x * blockDim.x;  // a multiple of 64
+    for (uint32_t vi = blockIdx.x * blockDim.x + threadIdx.x; vi < nsurv * 64; vi += stride) {
+        const uint32_t slot = perm ? perm[vi] : vi;
+        const bool victim = __any(pool[F_STOP_REL * a.cap + slot] != 0);
+        if ((threadIdx.x & 63) == 0) a.batch_victim[vi / 64] = victim ? 1u : 0u;
+    }
+}
+
 __device__ __forceinline__ void spawn_body(const Args &a) {
     DevState *st = a.st;
+    if (a.batch_victim) victim_flags(a, st);
     const uint32_t n = st->spawn_count;
     if (n == 0) return;
     const uint32_t want = st->want;
@@ -1057,6 +1074,68 @@ __device__ __forceinline__ void end_body(const Args &a) {
 }
 __global__ void k_end(Args a) { end_body(a); }
 
+// The per-block render's dispatch lists (one workgroup per XCD). The live
+// pool's survivor batches [0, nsurv) are split into XCDS contiguous runs, as
+// xcd_group splits them; the batches [nsurv, nb) that hold new voices are dealt
+// to the XCDs in turn. XCD x renders, in order: its new-voice batches, its
+// run's batches holding a steal victim, then its run's other batches, each
+// class walked from the run's end. New voices start late and victims fade, so
+// those waves run longest; started first, they no longer trail the launch.
+// Every batch appears in exactly one list once.
+__global__ void __launch_bounds__(1024) k_order(Args a) {
+    const DevState *st = a.st;
+    if (st->idle) return;
+    const uint32_t live_total = st->live_total;
+    const uint32_t nb = (live_total + 63) / 64;
+    const uint32_t nsurv = st->live / 64 < nb ? st->live / 64 : nb;
+    const uint32_t ns = nb - nsurv;
+    const uint32_t x = blockIdx.x;
+    const uint32_t q = nsurv / XCDS, r = nsurv % XCDS;
+    const uint32_t n = q + (x < r ? 1u : 0u);
+    const uint32_t r1 = x * q + (x < r ? x : r) + n;  // the run is [r1 - n, r1)
+    const uint32_t nsp = ns > x ? (ns - 1 - x) / XCDS + 1 : 0;
+    uint32_t *out = a.render_order + (size_t)x * a.order_stride;
+    const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6, nw = blockDim.x >> 6;
+    for (uint32_t k = t; k < nsp; k += blockDim.x) out[k] = nsurv + x + k * XCDS;
+    __shared__ uint32_t wsum[16], total;
+    // How many of the run's batches hold a victim.
+    uint32_t cnt = 0;
+    for (uint32_t j = t; j < n; j += blockDim.x) cnt += a.batch_victim[r1 - 1 - j] ? 1u : 0u;
+    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
+    if (lane == 0) wsum[wave] = cnt;
+    __syncthreads();
+    if (t == 0) {
+        uint32_t e = 0;
+        for (uint32_t w = 0; w < nw; ++w) e += wsum[w];
+        total = e;
+    }
+    __syncthreads();
+    const uint32_t nvict = total;
+    // Stable placement, chunk by chunk of the run walked from its end.
+    uint32_t before_e = 0;
+    for (uint32_t c0 = 0; c0 < n; c0 += blockDim.x) {
+        const uint32_t j = c0 + t;
+        const bool in = j < n;
+        const bool e = in && a.batch_victim[r1 - 1 - j] != 0;
+        const uint64_t m = __ballot(e);
+        const uint32_t below = __popcll(m & ((1ull << lane) - 1ull));
+        __syncthreads();  // wsum of the previous chunk is consumed
+        if (lane == 0) wsum[wave] = __popcll(m);
+        __syncthreads();
+        uint32_t wbase = 0, ctotal = 0;
+        for (uint32_t w = 0; w < nw; ++w) {
+            wbase += w < wave ? wsum[w] : 0u;
+            ctotal += wsum[w];
+        }
+        if (in) {
+            const uint32_t pe = before_e + wbase + below;  // victims before this one
+            const uint32_t pos = e ? nsp + pe : nsp + nvict + (j - pe);
+            out[pos] = r1 - 1 - j;
+        }
+        before_e += ctotal;
+    }
+}
+
 // A single submitted block's per-block inputs, from kernel arguments.
 __global__ void k_stage_one(Args a, uint32_t chan_flags, uint32_t ncmd) {
     const_cast<uint32_t *>(a.chan_flags)[0] = chan_flags;
@@ -1095,6 +1174,8 @@ void launch_select(const Args &a, uint32_t grid, uint32_t rounds, hipStream_t s)
 void launch_spawn(const Args &a, uint32_t grid, hipStream_t s) {
     k_spawn<<<grid, 256, 0, s>>>(a);
 }
+
+void launch_order(const Args &a, hipStream_t s) { k_order<<<XCDS, 1024, 0, s>>>(a); }
 
 void launch_sort(const Args &a, uint32_t grid, uint32_t gather_grid, hipStream_t s) {
     const uint32_t radix = 1u << a.digit_bits;

@@ -864,32 +864,14 @@ __device__ __forceinline__ void two_sum(float a, float b, float &s, float &e) {
 // One wave renders one batch of 64 voices for the whole block. Each 16-frame
 // sub-tile it reduces its 64 x 32 contributions through LDS and writes the 32
 // sums into its own row of the partial buffer (row = batch), so no two waves
-// share an accumulator and the summation order never depends on scheduling.
+// share an accumulator and the summation order never depends on scheduling,
+// nor on which wave renders which batch.
 template <int INTERP, bool CHAN, bool CLAMP>
-__device__ __forceinline__ void render_fast_body(const Args &a) {
-    extern __shared__ float lds[];
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t wave = threadIdx.x >> 6;
-    const uint32_t wpg = blockDim.x >> 6;
+__device__ __forceinline__ void render_batch(const Args &a, uint32_t batch, uint32_t live_total,
+                                             float *tb, const float *ftab, uint32_t lane) {
+    DevState *st = a.st;
     const uint32_t F = a.F;
     const uint32_t two_f = 2 * F;
-    float *tb = lds + wave * TBUF_FLOATS;
-    float *ftab = lds + wpg * TBUF_FLOATS;
-    for (uint32_t i = threadIdx.x; i < a.fade_lds; i += blockDim.x) ftab[i] = a.fade_tab[i];
-    __syncthreads();
-
-    DevState *st = a.st;
-    const uint32_t live_total = st->live_total;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        // What the next block's victim selection reads: the slots written here.
-        st->snap_n = live_total;
-        st->snap_parity = a.fused ? st->parity ^ 1 : st->parity;
-        st->snap_block = st->block;
-    }
-    const uint32_t groups = (live_total + 64 * wpg - 1) / (64 * wpg);
-    if (blockIdx.x >= groups) return;
-    const uint32_t batch = xcd_group_rev(blockIdx.x, groups) * wpg + wave;
-    if (batch * 64 >= live_total) return;
     const uint32_t b = st->block;
     const uint32_t chan_flags = CHAN ? st->chan_flags : 0u;
     uint32_t *pool = sel2(a.pools, st->parity);
@@ -963,17 +945,82 @@ __device__ __forceinline__ void render_fast_body(const Args &a) {
     }
     if (v.live) store_voice(a, dst, vi, v);
 }
-// At least three waves per SIMD (at most 168 VGPRs). Nearest and linear fit in
-// 128 and keep four; cubic's wider tap window takes 132-141, and three waves
-// without spills render 2.5 % faster than four with 4-14 VGPRs spilled.
+
+// The render's prologue: the fade table into LDS, and (one thread) what the
+// next block's victim selection reads, the slots this render writes.
+__device__ __forceinline__ uint32_t render_prologue(const Args &a, float *ftab) {
+    for (uint32_t i = threadIdx.x; i < a.fade_lds; i += blockDim.x) ftab[i] = a.fade_tab[i];
+    __syncthreads();
+    DevState *st = a.st;
+    const uint32_t live_total = st->live_total;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->snap_n = live_total;
+        st->snap_parity = a.fused ? st->parity ^ 1 : st->parity;
+        st->snap_block = st->block;
+    }
+    return live_total;
+}
+
+// The per-block render. Workgroup w runs on XCD x = w mod 8 (up to a rotation
+// that keeps the classes together) and each XCD starts its workgroups in
+// order, so workgroup w takes entries (w / 8) * wpg + wave of XCD x's dispatch
+// list (k_order): the XCD works through one contiguous run of the sorted pool
+// (its L2 holds the samples of a few regions), new voices and steal victims --
+// the longest waves -- first. Without the lists (render_order null) the same
+// runs are walked from their ends.
+// Waves per SIMD: nearest and linear fit in 128 VGPRs and keep four; cubic's
+// wider tap window takes more, and three waves without spills render 2.5 %
+// faster than four with 4-14 VGPRs spilled.
 template <int INTERP, bool CHAN, bool CLAMP>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
-k_render_fast(Args a) { render_fast_body<INTERP, CHAN, CLAMP>(a); }
-// The same pass over a batch of renders: blockIdx.z picks the render.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(INTERP == 2 ? 3 : 4)))
+k_render_fast(Args a) {
+    extern __shared__ float lds[];
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t wpg = blockDim.x >> 6;
+    float *tb = lds + wave * TBUF_FLOATS;
+    float *ftab = lds + wpg * TBUF_FLOATS;
+    const uint32_t live_total = render_prologue(a, ftab);
+    uint32_t batch;
+    if (a.render_order) {
+        const DevState *st = a.st;
+        const uint32_t nb = (live_total + 63) / 64;
+        const uint32_t nsurv = st->live / 64 < nb ? st->live / 64 : nb;
+        const uint32_t ns = nb - nsurv;
+        const uint32_t x = blockIdx.x % XCDS;
+        const uint32_t k = (blockIdx.x / XCDS) * wpg + wave;
+        const uint32_t len = nsurv / XCDS + (x < nsurv % XCDS ? 1u : 0u) +
+                             (ns > x ? (ns - 1 - x) / XCDS + 1 : 0u);
+        if (k >= len) return;
+        batch = a.render_order[(size_t)x * a.order_stride + k];
+    } else {
+        const uint32_t groups = (live_total + 64 * wpg - 1) / (64 * wpg);
+        if (blockIdx.x >= groups) return;
+        batch = xcd_group_rev(blockIdx.x, groups) * wpg + wave;
+        if (batch * 64 >= live_total) return;
+    }
+    render_batch<INTERP, CHAN, CLAMP>(a, batch, live_total, tb, ftab, lane);
+}
+
+// The same pass over a batch of renders: blockIdx.z picks the render, and
+// workgroup w renders the batches w * wpg + wave of its XCD's run, walked
+// from the run's end (new voices, the slowest waves, first).
 template <int INTERP, bool CHAN, bool CLAMP>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 kb_render_fast(const Args *__restrict__ aa) {
-    render_fast_body<INTERP, CHAN, CLAMP>(aa[blockIdx.z]);
+    extern __shared__ float lds[];
+    const Args &a = aa[blockIdx.z];
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t wpg = blockDim.x >> 6;
+    float *tb = lds + wave * TBUF_FLOATS;
+    float *ftab = lds + wpg * TBUF_FLOATS;
+    const uint32_t live_total = render_prologue(a, ftab);
+    const uint32_t groups = (live_total + 64 * wpg - 1) / (64 * wpg);
+    if (blockIdx.x >= groups) return;
+    const uint32_t batch = xcd_group_rev(blockIdx.x, groups) * wpg + wave;
+    if (batch * 64 >= live_total) return;
+    render_batch<INTERP, CHAN, CLAMP>(a, batch, live_total, tb, ftab, lane);
 }
 
 // Stage 1 of the mixdown: column sums of the batch rows in fixed order, one
@@ -1208,9 +1255,17 @@ __global__ void __launch_bounds__(1024) k_reduce_plain(Args a) {
 
 template <int INTERP, bool CHAN, bool CLAMP>
 void launch_fast_t(const Args &a, hipStream_t s) {
-    const uint32_t threads = a.waves_per_group * 64;
-    const size_t lds = fast_render_lds_bytes(a.waves_per_group, a.fade_lds);
-    k_render_fast<INTERP, CHAN, CLAMP><<<a.render_groups, threads, lds, s>>>(a);
+    const uint32_t wpg = a.waves_per_group;
+    const uint32_t threads = wpg * 64;
+    const size_t lds = fast_render_lds_bytes(wpg, a.fade_lds);
+    uint32_t grid = a.render_groups;
+    if (a.render_order) {
+        // Each XCD's list holds at most its run plus its share of the new
+        // voices' batches: ceil(batches / 8) + 2 entries.
+        const uint32_t per_xcd = (a.render_groups * wpg + XCDS - 1) / XCDS + 2;
+        grid = XCDS * ((per_xcd + wpg - 1) / wpg);
+    }
+    k_render_fast<INTERP, CHAN, CLAMP><<<grid, threads, lds, s>>>(a);
 }
 
 template <int INTERP, bool CHAN>
