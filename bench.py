@@ -135,20 +135,29 @@ WORKLOADS = {
 }
 
 
+def timed_profile(args) -> int:
+    """The timed run records HIP events around the render kernel only (its
+    duration is the roofline's denominator); events between every pass would
+    add a few microseconds per pass to the block. The per-pass breakdown comes
+    from a replay of the same blocks afterwards."""
+    from kestrel_amd.config import PROFILE_OFF, PROFILE_RENDER
+    return PROFILE_OFF if args.graph else PROFILE_RENDER
+
+
 def setup(name: str, args, seconds: float):
     """(cfg, bank, builder, design_on_device) for a named workload."""
     from kestrel_amd import workload as W
     if name == "c1":
-        cfg = W.config_c1(compensated=bool(args.compensated), profile=not args.graph)
+        cfg = W.config_c1(compensated=bool(args.compensated), profile=timed_profile(args))
         cfg, bank, notes = W.workload_c1(cfg=cfg)
         return cfg, bank, W.BlockBuilder(cfg, bank, notes), False
     if name == "c2":
-        cfg = W.config_c2(compensated=bool(args.compensated), profile=not args.graph)
+        cfg = W.config_c2(compensated=bool(args.compensated), profile=timed_profile(args))
         _, bank, notes = W.workload_c2(args.seed or 0x4B530002, seconds=seconds, cfg=cfg,
                                        base_rate_hz=500_000.0)
         return cfg, bank, W.BlockBuilder(cfg, bank, notes), False
     if name == "c3":
-        cfg = W.config_c3(profile=not args.graph)
+        cfg = W.config_c3(profile=timed_profile(args))
         _, bank, notes = W.workload_c3(args.seed or 0x4B530003, seconds=seconds, cfg=cfg,
                                        base_rate_hz=500_000.0)
         return cfg, bank, W.BlockBuilder(cfg, bank, notes,
@@ -756,9 +765,24 @@ def main():
     log(f"timed: {K} blocks in {wall * 1e3:.1f} ms ({wall / K * 1e3:.3f} ms/block), "
         f"mean voices {V.mean():.0f}, {value:.3e} voice-samples/s, {x_rt:.1f}x realtime, "
         f"blocks with steals {n_steal}/{K}")
-    log(f"per-pass ms/block: {timings}")
+    log(f"render ms/block (events around the render kernel): {timings}")
     pool_final = gpu.pool_export()
     saturated = (not saturate) or (float(V.mean()) >= cfg.max_voices and n_steal == K)
+    # ---- per-pass breakdown: the first timed blocks again from the snapshot,
+    # with events between every pass (outside the timed region) ----
+    passes = None
+    if not args.graph:
+        from kestrel_amd.config import PROFILE_ALL, PROFILE_RENDER
+        nb_b = min(K, 64)
+        gpu.pool_import(snapshot)
+        gpu.set_counters(st0["stolen"], st0["dropped"])
+        stage(gpu, timed[:nb_b])
+        gpu.set_profile(PROFILE_ALL)
+        gpu.run_staged(False)
+        passes = {k: round(v, 4) for k, v in gpu.timings()}
+        gpu.set_profile(PROFILE_RENDER)
+        log(f"per-pass ms/block (replay of {nb_b} timed blocks with events between passes): "
+            f"{passes}")
 
     # ---- roofline of the render kernel ----
     step = (snapshot[3].astype(np.float64) * 2 ** 32 + snapshot[2]) / 2 ** 32
@@ -898,6 +922,7 @@ def main():
                    "form": gpu.name(), "graph": bool(args.graph)},
         "x_realtime": round(x_rt, 2),
         "roofline": roofline,
+        "passes_ms": passes,
         "cpu_baseline": cpu_baseline,
         "plain": plain,
         "vs_plain": round(value / plain["value"], 3) if plain else None,

@@ -29,6 +29,11 @@ extern "C" {
 #define KST_EDEVICE (-2)  /* a HIP runtime failure */
 #define KST_ESTATE (-3)   /* called out of order */
 
+/* kst_config.profile levels. */
+#define KST_PROFILE_OFF 0u
+#define KST_PROFILE_ALL 1u     /* events between every pass of every block */
+#define KST_PROFILE_RENDER 2u  /* events around the render kernel only */
+
 /* One voice's birth record. src/voice.rs:22-57, 76 bytes, same field order. */
 typedef struct kst_spawn_cmd {
     uint32_t phase_lo, phase_hi, step_lo, step_hi;
@@ -89,7 +94,8 @@ typedef struct kst_config {
     uint32_t waves_per_group;       /* wave64 form: waves per render workgroup (1..4) */
     uint32_t sort_digit_bits;       /* radix width of the locality sort, 1..8 */
     int32_t device;                 /* HIP device ordinal */
-    uint32_t profile;               /* per-pass timings (src/config.rs:337-338) */
+    uint32_t profile;               /* per-pass timings (src/config.rs:337-338):
+                                       KST_PROFILE_OFF, _ALL or _RENDER */
 } kst_config;
 
 typedef struct kst_backend kst_backend;
@@ -145,8 +151,16 @@ const char *kst_name(const kst_backend *b);
 
 /* Backend::timings -- src/backend.rs:53-57. Writes up to `cap` entries and
  * returns how many there are. Empty until a block has rendered with `profile`
- * on; after a staged run without a graph, each pass's mean over its blocks. */
+ * on; after a staged run without a graph, each pass's mean over its blocks
+ * (under KST_PROFILE_RENDER the render kernel's only). */
 size_t kst_timings(const kst_backend *b, kst_timing *out, size_t cap);
+
+/* Change the profile level of a backend between blocks (KST_PROFILE_OFF,
+ * KST_PROFILE_ALL: events between every pass, KST_PROFILE_RENDER: events
+ * around the render kernel only, which leaves the rest of the chain as
+ * without profiling). No counterpart in the reference: the config's
+ * `profile` fixes it for the backend's life there (src/config.rs:337-338). */
+int kst_set_profile(kst_backend *b, uint32_t level);
 
 /* The message of the last failure on this thread (anyhow's error text). */
 const char *kst_last_error(void);

@@ -111,6 +111,7 @@ SIGNATURES = {
     "kst_stats": (i32, [P, P]),
     "kst_name": (C.c_char_p, [P]),
     "kst_timings": (sz, [P, P, sz]),
+    "kst_set_profile": (i32, [P, u32]),
     "kst_last_error": (C.c_char_p, []),
     "kst_stage_blocks": (i32, [P, u32, P, P, P, P, P]),
     "kst_run_staged": (i32, [P, i32]),

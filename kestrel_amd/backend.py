@@ -101,6 +101,11 @@ class HipSynth:
     def name(self) -> str:
         return self._lib.kst_name(self._h).decode()
 
+    def set_profile(self, level: int) -> None:
+        """PROFILE_OFF, PROFILE_ALL (events between every pass) or
+        PROFILE_RENDER (events around the render kernel only)."""
+        abi.check(self._lib.kst_set_profile(self._h, int(level)))
+
     def timings(self) -> list[tuple[str, float]]:
         buf = (abi.KstTiming * 16)()
         n = self._lib.kst_timings(self._h, buf, 16)

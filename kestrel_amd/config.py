@@ -17,6 +17,9 @@ STEAL_OLDEST, STEAL_DROP_NEW, STEAL_QUIETEST = 0, 1, 2
 ADMIT_LOUDEST, ADMIT_EVEN = 0, 1
 
 FORM_PLAIN, FORM_WAVE64 = 0, 1
+# kst_config.profile: off, HIP events between every pass, or around the render
+# kernel only (the rest of the block chain runs as unprofiled).
+PROFILE_OFF, PROFILE_ALL, PROFILE_RENDER = 0, 1, 2
 
 
 class ConfigError(ValueError):
@@ -66,7 +69,7 @@ class Config:
     waves_per_group: int = 1
     sort_digit_bits: int = 9
     device: int = 0
-    profile: bool = False
+    profile: int = 0  # PROFILE_OFF, PROFILE_ALL or PROFILE_RENDER
 
     def validate(self) -> None:
         """src/config.rs:412-525, the render-path subset."""

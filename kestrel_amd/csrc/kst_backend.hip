@@ -220,7 +220,8 @@ struct kst_backend {
     uint64_t stolen = 0, dropped = 0;
     float peak = 0.0f;
 
-    bool profile = false;
+    // 0 off, 1 every pass, 2 the render kernel only (KST_PROFILE_*).
+    uint32_t profile = 0;
     bool inflight = false;  // a render submitted and not yet collected
     // Select rounds launched per block. A round whose bin is already small
     // enough returns at once; rounds still needed after these are finished by
@@ -432,7 +433,11 @@ struct kst_backend {
     // this one reduces, sorts and compacts, and the block ends by joining it.
     void launch_block(uint32_t n_live, uint32_t n_total, bool may_steal, bool spawn, bool chan,
                       hipEvent_t *ev, bool pipelined = false) {
-        const bool timed = ev != nullptr;
+        // Events around every pass, or (KST_PROFILE_RENDER) only around the
+        // render kernel: each event in the stream costs the chain a few
+        // microseconds.
+        const bool timed_render = ev != nullptr;
+        const bool timed = timed_render && profile != KST_PROFILE_RENDER;
         const uint32_t rg = render_groups_for(n_total);
         Args a = args(rg);
         hipStream_t s = stream;
@@ -446,12 +451,17 @@ struct kst_backend {
         // from which k_order builds the render's dispatch lists.
         if (spawn || fast) launch_spawn(a, per_voice_grid(n_total), s);
         if (fast) launch_order(a, s);
-        if (timed) (void)hipEventRecord(ev[2], s);
-        if (fast)
-            launch_render_fast(a, chan, s);
-        else
-            launch_render_plain(a, chan, s);
-        if (timed) (void)hipEventRecord(ev[3], s);
+        if (fast && timed_render && !timed) {
+            // The render kernel alone: its launch records its own start and end.
+            launch_render_fast(a, chan, s, ev[2], ev[3]);
+        } else {
+            if (timed_render) (void)hipEventRecord(ev[2], s);
+            if (fast)
+                launch_render_fast(a, chan, s);
+            else
+                launch_render_plain(a, chan, s);
+            if (timed_render) (void)hipEventRecord(ev[3], s);
+        }
         if (pipelined) {
             (void)hipEventRecord(ev_rendered, s);
             (void)hipStreamWaitEvent(side, ev_rendered, 0);
@@ -470,10 +480,12 @@ struct kst_backend {
         if (timed) (void)hipEventRecord(ev[5], s);
     }
 
-    // Per-pass milliseconds, averaged over `nblocks` event sets of 6.
+    // Per-pass milliseconds, averaged over `nblocks` event sets of 6 (only
+    // the render's pair under KST_PROFILE_RENDER).
     void read_timings(const hipEvent_t *evs, uint32_t nblocks) {
         timings.clear();
         for (int i = 0; i < 5; ++i) {
+            if (profile == KST_PROFILE_RENDER && i != 2) continue;
             double sum = 0.0;
             for (uint32_t k = 0; k < nblocks; ++k) {
                 float ms = 0.0f;
@@ -718,7 +730,8 @@ int kst_create(const kst_config *cfg, const int16_t *pool, size_t pool_len,
     for (auto &x : b->ev)
         if (hipEventCreate(&x) != hipSuccess)
             return cleanup(fail(KST_EDEVICE, "creating timing events"));
-    b->profile = c.profile != 0 || getenv("KST_PROFILE") != nullptr;
+    b->profile = c.profile <= KST_PROFILE_RENDER ? c.profile : KST_PROFILE_ALL;
+    if (b->profile == 0 && getenv("KST_PROFILE") != nullptr) b->profile = KST_PROFILE_ALL;
     *out = b;
     return KST_OK;
 }
@@ -937,6 +950,18 @@ int kst_stats(const kst_backend *b, kst_block_stats *out) {
 const char *kst_name(const kst_backend *b) {
     if (!b) return "";
     return b->fast ? (b->cfg.compensated ? "hip-wave64-compensated" : "hip-wave64") : "hip-plain";
+}
+
+int kst_set_profile(kst_backend *b, uint32_t level) {
+    if (!b) return fail(KST_EINVAL, "null backend");
+    if (level > KST_PROFILE_RENDER)
+        return fail(KST_EINVAL, "profile level %u must be 0, 1 (every pass) or 2 (render only)",
+                    level);
+    if (b->inflight) return fail(KST_ESTATE, "a render is in flight");
+    b->profile = level;
+    b->cfg.profile = level;
+    b->timings.clear();
+    return KST_OK;
 }
 
 size_t kst_timings(const kst_backend *b, kst_timing *out, size_t cap) {

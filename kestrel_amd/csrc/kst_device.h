@@ -236,7 +236,10 @@ void launch_settle(const Args &a, uint32_t gather_grid, hipStream_t s);
 // Returns false when the wave64 form does not cover this configuration.
 bool fast_render_supported(uint32_t F, uint32_t gate_frames, uint32_t interp);
 size_t fast_render_lds_bytes(uint32_t waves_per_group, uint32_t fade_lds);
-void launch_render_fast(const Args &a, bool chan, hipStream_t s);
+// With `start`/`stop`, the launch itself records the kernel's start and end
+// (hipExtLaunchKernel): no event packets of their own in the stream.
+void launch_render_fast(const Args &a, bool chan, hipStream_t s, hipEvent_t start = nullptr,
+                        hipEvent_t stop = nullptr);
 
 // A batch of renders in one launch per pass: `aa` is a device array of `nr`
 // Args (the render's own buffers and config), and blockIdx.z picks the

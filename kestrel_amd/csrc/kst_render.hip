@@ -22,6 +22,8 @@
 //
 // No float atomics anywhere; every summation order is fixed, so two runs are
 // bitwise identical.
+#include <hip/hip_ext.h>
+
 #include "kst_device.h"
 
 // The structured buffer load (LLVM's llvm.amdgcn.struct.buffer.load.i32, which
@@ -1254,7 +1256,7 @@ __global__ void __launch_bounds__(1024) k_reduce_plain(Args a) {
 }
 
 template <int INTERP, bool CHAN, bool CLAMP>
-void launch_fast_t(const Args &a, hipStream_t s) {
+void launch_fast_t(const Args &a, hipStream_t s, hipEvent_t start, hipEvent_t stop) {
     const uint32_t wpg = a.waves_per_group;
     const uint32_t threads = wpg * 64;
     const size_t lds = fast_render_lds_bytes(wpg, a.fade_lds);
@@ -1265,7 +1267,11 @@ void launch_fast_t(const Args &a, hipStream_t s) {
         const uint32_t per_xcd = (a.render_groups * wpg + XCDS - 1) / XCDS + 2;
         grid = XCDS * ((per_xcd + wpg - 1) / wpg);
     }
-    k_render_fast<INTERP, CHAN, CLAMP><<<grid, threads, lds, s>>>(a);
+    if (start || stop)
+        hipExtLaunchKernelGGL(k_render_fast<INTERP, CHAN, CLAMP>, dim3(grid), dim3(threads),
+                              (uint32_t)lds, s, start, stop, 0u, a);
+    else
+        k_render_fast<INTERP, CHAN, CLAMP><<<grid, threads, lds, s>>>(a);
 }
 
 template <int INTERP, bool CHAN>
@@ -1285,10 +1291,12 @@ size_t fast_render_lds_bytes(uint32_t waves_per_group, uint32_t fade_lds) {
     return ((size_t)waves_per_group * TBUF_FLOATS + ((fade_lds + 3) & ~3u)) * sizeof(float);
 }
 
-void launch_render_fast(const Args &a, bool chan, hipStream_t s) {
+void launch_render_fast(const Args &a, bool chan, hipStream_t s, hipEvent_t start,
+                        hipEvent_t stop) {
 #define KST_FAST(I, CL)                                                                    \
     if (a.interp == I && (a.clamp != 0) == CL) {                                           \
-        chan ? launch_fast_t<I, true, CL>(a, s) : launch_fast_t<I, false, CL>(a, s);       \
+        chan ? launch_fast_t<I, true, CL>(a, s, start, stop)                               \
+             : launch_fast_t<I, false, CL>(a, s, start, stop);                             \
         return;                                                                            \
     }
     KST_FAST(0, false)

@@ -321,6 +321,37 @@ def test_validation_errors_name_the_flag():
     g.close()
 
 
+def test_profile_levels_time_the_passes_they_name():
+    """PROFILE_RENDER times the render kernel only, PROFILE_ALL every pass;
+    either way the staged blocks' outputs are the unprofiled ones."""
+    from kestrel_amd.abi import BackendError
+    from kestrel_amd.backend import HipSynth
+    from kestrel_amd.config import PROFILE_ALL, PROFILE_OFF, PROFILE_RENDER
+    cfg, bank, notes = W.workload_c1()
+    outs = {}
+    for level in (PROFILE_OFF, PROFILE_RENDER, PROFILE_ALL):
+        g = HipSynth(cfg.replace(profile=level), bank.pool, bank.params, bank.n_regions,
+                     bank.max_sample_len)
+        bb = W.BlockBuilder(cfg, bank, notes)
+        blocks = [bb.next_block() for _ in range(12)]
+        g.stage_blocks(np.concatenate([b.gates for b in blocks]),
+                       np.concatenate([b.chans for b in blocks]),
+                       np.array([b.chan_flag_word for b in blocks], np.uint32),
+                       np.concatenate([b.cmds for b in blocks]),
+                       np.array([len(b.cmds) for b in blocks], np.uint32))
+        g.run_staged(False)
+        outs[level], _ = g.staged_results()
+        names = [n for n, _ in g.timings()]
+        assert names == {PROFILE_OFF: [], PROFILE_RENDER: ["render"],
+                         PROFILE_ALL: ["steal", "spawn", "render", "reduce", "compact"]}[level]
+        assert all(ms > 0 for _, ms in g.timings())
+        with pytest.raises(BackendError, match="profile level"):
+            g.set_profile(3)
+        g.close()
+    assert np.array_equal(outs[PROFILE_OFF], outs[PROFILE_RENDER])
+    assert np.array_equal(outs[PROFILE_OFF], outs[PROFILE_ALL])
+
+
 @pytest.mark.parametrize("rule,digit_bits", [(2, 8), (0, 9), (2, 5)])
 def test_multi_tile_sort_and_steal_at_40k_voices(rule, digit_bits):
     """A pool large enough for many sort tiles and select workgroups, saturated
