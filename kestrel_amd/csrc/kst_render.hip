@@ -202,6 +202,66 @@ __device__ __forceinline__ bool gate_tile(const Args &a, const uint32_t *gates,
     return changed;
 }
 
+// Gate tiles whose rows one load round covers (the wave64 render).
+constexpr uint32_t GATE_WINDOW = 16;
+
+// The first gate tile in [g0, g0 + GATE_WINDOW) whose row releases this
+// lane's note (gates[tile][slot] >= ordinal, src/cpu.rs:449-455), or
+// g0 + GATE_WINDOW. Stages only advance, so a note is released at the first
+// such tile if at all: once the stage reaches RELEASE later rows change
+// nothing. The window's rows are loaded together, so a block of up to
+// GATE_WINDOW gate tiles waits on its gate rows once.
+__device__ __forceinline__ uint32_t gate_window(const uint32_t *gates, uint32_t g0,
+                                               uint32_t tiles, const Voice &v) {
+    const uint32_t slot = v.gate_slot & (GATE_SLOTS - 1);
+    uint32_t row[GATE_WINDOW];
+#pragma unroll
+    for (uint32_t k = 0; k < GATE_WINDOW; ++k)
+        row[k] = g0 + k < tiles ? gates[(g0 + k) * GATE_SLOTS + slot] : 0u;
+    uint32_t first = g0 + GATE_WINDOW;
+#pragma unroll
+    for (uint32_t k = GATE_WINDOW; k-- > 0;)
+        if (g0 + k < tiles && row[k] >= v.ordinal) first = g0 + k;
+    return first;
+}
+
+// gate_tile with the note-off decided by gate_window: `release` is this
+// lane's first releasing tile.
+template <bool CHAN>
+__device__ __forceinline__ bool gate_tile_at(const Args &a, const uint32_t *chan,
+                                             uint32_t chan_flags, uint32_t gt, uint32_t release,
+                                             Voice &v) {
+    if (!v.live) return false;
+    bool changed = false;
+    if (v.stage < ENV_RELEASE && gt == release) {
+        v.stage = ENV_RELEASE;
+        v.level = fminf(v.level, 1.0f);
+        changed = true;
+    }
+    if (CHAN && chan_flags) {
+        const uint32_t channel = (v.gate_slot >> 7) & 15;
+        const uint32_t ci = (gt * BEND_CHANNELS + channel) * CHAN_FIELDS;
+        if (chan_flags & CHAN_ACTIVE_BEND) {
+            v.step = fixed_scale(v.base_step, chan[ci + CHAN_BEND]);
+            changed = true;
+        }
+        if (chan_flags & CHAN_ACTIVE_GAIN) {
+            v.gl = v.bgl * __uint_as_float(chan[ci + CHAN_GAIN_L]);
+            v.gr = v.bgr * __uint_as_float(chan[ci + CHAN_GAIN_R]);
+        }
+        if (chan_flags & CHAN_ACTIVE_VARIANT) {
+            const uint32_t want = chan[ci + CHAN_VARIANT];
+            const uint32_t born_tile = v.start_rel / a.gate_frames;
+            const bool born_here = v.born_variant != 0 && gt <= born_tile;
+            if (want != v.variant && !born_here) {
+                v.variant = want;
+                v.p = load_params(a, v.variant, v.params_base);
+            }
+        }
+    }
+    return changed;
+}
+
 __device__ __forceinline__ float fetch(const Args &a, uint32_t base, uint32_t idx) {
     // src/cpu.rs:190-194. u32 index arithmetic as in the reference; out of the
     // pool reads zero.
@@ -897,11 +957,19 @@ __device__ __forceinline__ void render_batch(const Args &a, uint32_t batch, uint
     load_voice(a, src, slot, in, v);
     if (a.fused && in && survivor) copy_static(a, src, slot, dst, vi, v);
     uint64_t lim = fast_limit<INTERP>(a, v);
+    const uint32_t tiles = F / a.gate_frames;
+    uint32_t release = gate_window(gates, 0, tiles, v), window_end = GATE_WINDOW;
     for (uint32_t s = 0; s < F / SUBTILE; ++s) {
         const uint32_t f0 = s * SUBTILE;
-        if (f0 % a.gate_frames == 0 &&
-            gate_tile<CHAN>(a, gates, chan, chan_flags, f0 / a.gate_frames, v))
-            lim = fast_limit<INTERP>(a, v);
+        if (f0 % a.gate_frames == 0) {
+            const uint32_t gt = f0 / a.gate_frames;
+            if (gt >= window_end) {
+                release = gate_window(gates, gt, tiles, v);
+                window_end = gt + GATE_WINDOW;
+            }
+            if (gate_tile_at<CHAN>(a, chan, chan_flags, gt, release, v))
+                lim = fast_limit<INTERP>(a, v);
+        }
         const bool fok = fast_ok<INTERP>(v, lim, f0);
         const bool straight = !__any(!fok);
         const bool fading = __any(fade_in(v, f0));
