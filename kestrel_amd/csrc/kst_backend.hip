@@ -705,14 +705,17 @@ int kst_create(const kst_config *cfg, const int16_t *pool, size_t pool_len,
     // The steal fade's gain for d = 0 .. fade-1 (src/cpu.rs:577-586): the same
     // correctly rounded f32 division and subtraction the per-frame path does.
     {
+        // Padded with FADE_PAD gains of 1 before and of 0 after, the frames
+        // before a steal's fade and from its last on.
         const uint32_t fade = c.steal_fade_frames;
-        std::vector<float> tab(fade);
-        for (uint32_t d = 0; d < fade; ++d) tab[d] = 1.0f - (float)d / (float)fade;
-        if (b->fade_tab.alloc(fade) != hipSuccess ||
-            hipMemcpy(b->fade_tab.p, tab.data(), fade * sizeof(float), hipMemcpyHostToDevice) !=
-                hipSuccess)
+        std::vector<float> tab(fade + 2 * FADE_PAD, 0.0f);
+        for (uint32_t d = 0; d < FADE_PAD; ++d) tab[d] = 1.0f;
+        for (uint32_t d = 0; d < fade; ++d) tab[FADE_PAD + d] = 1.0f - (float)d / (float)fade;
+        if (b->fade_tab.alloc(tab.size()) != hipSuccess ||
+            hipMemcpy(b->fade_tab.p, tab.data(), tab.size() * sizeof(float),
+                      hipMemcpyHostToDevice) != hipSuccess)
             return cleanup(fail(KST_EDEVICE, "uploading the fade table"));
-        b->fade_lds = fade <= FADE_LDS_MAX ? fade : 0;
+        b->fade_lds = fade <= FADE_LDS_MAX ? (uint32_t)tab.size() : 0;
     }
     DevState zero{};
     if (hipMemcpy(b->st.p, &zero, sizeof zero, hipMemcpyHostToDevice) != hipSuccess)

@@ -68,6 +68,9 @@ constexpr uint32_t SORT_MAX_RADIX = 1u << SORT_MAX_BITS;
 constexpr uint32_t SUBTILE = 16;
 constexpr uint32_t TROW = 68;
 constexpr uint32_t FADE_LDS_MAX = 2048;
+// The staged fade table: FADE_PAD gains of 1 before the fade's and FADE_PAD of
+// 0 after them, so any sub-tile reads it at 16 consecutive entries.
+constexpr uint32_t FADE_PAD = 16;
 constexpr uint32_t TBUF_FLOATS = 2 * SUBTILE * TROW;  // 32 rows
 // Batch rows summed per workgroup in the first mixdown stage.
 constexpr uint32_t REDUCE_ROWS = 256;
@@ -158,7 +161,9 @@ struct Args {
     // Config.
     uint32_t nblocks;  // block slots holding inputs; a block at or past this idles
     uint32_t F, gate_frames, max_voices, max_steal, steal_span, fade;
-    const float *fade_tab;  // [fade]: 1 - d / fade, as the per-frame fade computes it
+    // [FADE_PAD + fade + FADE_PAD]: 1 ..., then 1 - d / fade for d = 0 .. fade - 1
+    // as the per-frame fade computes it, then 0 ...
+    const float *fade_tab;
     uint32_t fade_lds;      // entries of fade_tab staged in LDS (0: too long to stage)
     uint32_t steal_rule, admit_rule, interp, exp_decay, exp_release;
     float env_floor;

@@ -652,10 +652,19 @@ __device__ __forceinline__ void fast_subtile(const Args &a, Voice &v, float *tb,
     const bool filt = FILT == 1 || (FILT == 2 && (v.p.flags & RP_FILTER));
     float g = fminf(v.level, 1.0f);
     bool kill = false;
-    const bool fades = FADE && v.stop_rel != 0;
-    int32_t d = (int32_t)(f0 + 1) - (int32_t)v.stop_rel;
-    const int32_t fade = (int32_t)a.fade;
+    // The steal fade: frame f0 + i is scaled by ftab[d0 + i], d0 = f0 + 1 -
+    // stop_rel, where the table (FADE_PAD entries of 1 before it, of 0 after
+    // it) gives 1 before the fade, 1 - d / fade during it and 0 from d = fade
+    // on, the frame that kills the voice. A lane with no fade reads ones.
+    const float *gtab = ftab;
     bool died = false;
+    if (FADE) {
+        const int32_t fade = (int32_t)a.fade;
+        const int32_t pad = (int32_t)FADE_PAD;
+        const int32_t d0 = v.stop_rel != 0 ? (int32_t)(f0 + 1) - (int32_t)v.stop_rel : -pad;
+        gtab = ftab + (d0 < -pad ? -pad : (d0 > fade ? fade : d0));
+        died = v.stop_rel != 0 && d0 + (int32_t)SUBTILE - 1 >= fade;
+    }
     // Cubic (and the wrap variant) runs its own 32-bit chain of the phase's
     // low word.
     uint32_t lo32 = (uint32_t)v.phase;
@@ -708,14 +717,13 @@ __device__ __forceinline__ void fast_subtile(const Args &a, Voice &v, float *tb,
             v.z2 = b0x - v.p.a2 * y;
         }
         if (FADE) {
-            const bool fading = fades && d >= 0;
-            const bool gone = fading && d >= fade;
-            const float fg = ftab[d < 0 ? 0 : (d < fade ? d : fade - 1)];
-            y = (fading && !gone) ? y * fg : y;
+            // y * 1 is y itself; a zero factor marks the frame that kills the
+            // voice, which sounds nothing (src/cpu.rs:577-586).
+            const float fg = gtab[i];
+            const bool gone = fg == 0.0f;
+            y = y * fg;
             tb[(2 * i) * TROW + lane] = gone ? 0.0f : y * v.gl;
             tb[(2 * i + 1) * TROW + lane] = gone ? 0.0f : y * v.gr;
-            died |= gone;
-            ++d;
         } else {
             tb[(2 * i) * TROW + lane] = y * v.gl;
             tb[(2 * i + 1) * TROW + lane] = y * v.gr;
@@ -740,7 +748,7 @@ __device__ __forceinline__ void fast_subtile_dispatch(const Args &a, Voice &v, f
                                                       const float *ftab, uint32_t lane,
                                                       uint32_t f0) {
     const bool sounding = v.live && v.stage != ENV_DEAD;
-    const bool env_static = !FADE && !__any(sounding && v.stage != ENV_SUSTAIN);
+    const bool env_static = !__any(sounding && v.stage != ENV_SUSTAIN);
     const bool f = (v.p.flags & RP_FILTER) != 0;
     const bool all_filtered = !__any(sounding && !f);
     const bool none_filtered = !__any(sounding && f);
@@ -749,7 +757,7 @@ __device__ __forceinline__ void fast_subtile_dispatch(const Args &a, Voice &v, f
     straight_taps<INTERP>(a, v, w0, w1, lo);
     if (all_filtered) {
         if (env_static)
-            fast_subtile<INTERP, CLAMP, 1, true, false>(a, v, tb, ftab, lane, f0, w0, w1, lo);
+            fast_subtile<INTERP, CLAMP, 1, true, FADE>(a, v, tb, ftab, lane, f0, w0, w1, lo);
         else
             fast_subtile<INTERP, CLAMP, 1, false, FADE>(a, v, tb, ftab, lane, f0, w0, w1, lo);
     } else if (none_filtered) {
@@ -977,11 +985,16 @@ __device__ __forceinline__ void render_batch(const Args &a, uint32_t batch, uint
             fast_subtile_dispatch<INTERP, CLAMP, false>(a, v, tb, ftab, lane, f0);
         } else if (straight && a.fade_lds != 0) {
             fast_subtile_dispatch<INTERP, CLAMP, true>(a, v, tb, ftab, lane, f0);
-        } else if (!fading && !__any(!(fok || wrap_ok<INTERP>(a, v, f0)))) {
+        } else if ((!fading || a.fade_lds != 0) && !__any(!(fok || wrap_ok<INTERP>(a, v, f0)))) {
             neutralise<INTERP>(v);
             uint32_t w0[SUBTILE], w1[INTERP == 2 ? SUBTILE : 1];
             const uint64_t end = wrap_taps<INTERP>(a, v, w0, w1);
-            fast_subtile<INTERP, CLAMP, 2, false, false, true>(a, v, tb, ftab, lane, f0, w0, w1, nullptr, end);
+            if (fading)
+                fast_subtile<INTERP, CLAMP, 2, false, true, true>(a, v, tb, ftab, lane, f0, w0, w1,
+                                                                  nullptr, end);
+            else
+                fast_subtile<INTERP, CLAMP, 2, false, false, true>(a, v, tb, ftab, lane, f0, w0, w1,
+                                                                   nullptr, end);
         } else if (INTERP != 2 && !__any(!edge_ok<INTERP>(a, v, f0))) {
             fast_subtile_edge<INTERP, CLAMP>(a, v, tb, ftab, lane, f0);
         } else {
@@ -1051,6 +1064,7 @@ k_render_fast(Args a) {
     float *tb = lds + wave * TBUF_FLOATS;
     float *ftab = lds + wpg * TBUF_FLOATS;
     const uint32_t live_total = render_prologue(a, ftab);
+    ftab += FADE_PAD;  // the fade gains, d = 0 .. fade - 1
     uint32_t batch;
     if (a.render_order) {
         const DevState *st = a.st;
@@ -1086,6 +1100,7 @@ kb_render_fast(const Args *__restrict__ aa) {
     float *tb = lds + wave * TBUF_FLOATS;
     float *ftab = lds + wpg * TBUF_FLOATS;
     const uint32_t live_total = render_prologue(a, ftab);
+    ftab += FADE_PAD;  // the fade gains, d = 0 .. fade - 1
     const uint32_t groups = (live_total + 64 * wpg - 1) / (64 * wpg);
     if (blockIdx.x >= groups) return;
     const uint32_t batch = xcd_group_rev(blockIdx.x, groups) * wpg + wave;
