@@ -240,6 +240,7 @@ struct kst_backend {
     // current block reduces and sorts (default priority, like the main stream).
     hipStream_t side = nullptr;
     hipEvent_t ev_rendered = nullptr, ev_selected = nullptr;
+    uint32_t sync_seq = 0;  // last sequence number of a device-ordered block
 
     ~kst_backend() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -417,9 +418,13 @@ struct kst_backend {
 
     // Victim selection for the block about to run, or (ahead) for the block
     // after the last render.
-    void launch_sel(uint32_t n_live, bool ahead, hipStream_t s) {
+    // `sync`: this selection's sequence number in a device-ordered staged run
+    // (it waits for the render's and publishes its own), 0 otherwise.
+    void launch_sel(uint32_t n_live, bool ahead, hipStream_t s, uint32_t sync = 0) {
         Args a = args(1);
         a.sel_ahead = ahead ? 1u : 0u;
+        a.wait_render = sync;
+        a.sig_sel = sync;
         launch_select(a, std::max(1u, std::min(std::min(cus, SEL_MAX_BLOCKS),
                                                (n_live + SEL_THREADS * 4 - 1) / (SEL_THREADS * 4))),
                       sel_rounds, s);
@@ -431,8 +436,16 @@ struct kst_backend {
     // runs, wave64 form): this block's victims were selected during the last
     // block; after the render the side stream selects the next block's while
     // this one reduces, sorts and compacts, and the block ends by joining it.
+    //
+    // device_sync (pipelined, not captured): the two streams are ordered on
+    // the device rather than by events, whose packets cost the chain ~20 us
+    // per fork and join. The mixdown after the render publishes the block's
+    // sequence number, the side stream's selection waits for it and publishes
+    // its own at its end, and k_end waits for that. Every wait is launched
+    // after the kernel it waits for, so the chain cannot deadlock even where
+    // both streams share one hardware queue, and every wait is bounded.
     void launch_block(uint32_t n_live, uint32_t n_total, bool may_steal, bool spawn, bool chan,
-                      hipEvent_t *ev, bool pipelined = false) {
+                      hipEvent_t *ev, bool pipelined = false, bool device_sync = false) {
         // Events around every pass, or (KST_PROFILE_RENDER) only around the
         // render kernel: each event in the stream costs the chain a few
         // microseconds.
@@ -462,21 +475,30 @@ struct kst_backend {
                 launch_render_plain(a, chan, s);
             if (timed_render) (void)hipEventRecord(ev[3], s);
         }
-        if (pipelined) {
+        const bool dsync = pipelined && device_sync;
+        uint32_t seq = 0;
+        if (dsync) {
+            seq = ++sync_seq;
+            if (seq == 0) seq = ++sync_seq;  // 0 means "no wait"
+        } else if (pipelined) {
             (void)hipEventRecord(ev_rendered, s);
             (void)hipStreamWaitEvent(side, ev_rendered, 0);
             launch_sel(n_live, true, side);
             (void)hipEventRecord(ev_selected, side);
         }
+        Args ar = a;
+        ar.sig_render = seq;
         if (fast)
-            launch_reduce_fast(a, cfg.compensated != 0, (n_total + 63) / 64, s);
+            launch_reduce_fast(ar, cfg.compensated != 0, (n_total + 63) / 64, s);
         else
             launch_reduce_plain(a, s);
+        if (dsync) launch_sel(n_live, true, side, seq);
         if (timed) (void)hipEventRecord(ev[4], s);
         launch_sort(a, sort_grid(n_total), per_voice_grid(n_total), s);
         if (pipelined) a.end_begins = 1;
+        a.wait_sel = seq;
         launch_end(a, s);
-        if (pipelined) (void)hipStreamWaitEvent(s, ev_selected, 0);
+        if (pipelined && !dsync) (void)hipStreamWaitEvent(s, ev_selected, 0);
         if (timed) (void)hipEventRecord(ev[5], s);
     }
 
@@ -1017,6 +1039,7 @@ int kst_run_staged(kst_backend *b, int use_graph) {
     hipStream_t s = b->stream;
     uint32_t zero = 0;
     HIPCHK(hipMemcpyAsync(&b->st.p->block, &zero, 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(&b->st.p->sync_timeout, &zero, 4, hipMemcpyHostToDevice, s));
     HIPCHK(hipStreamSynchronize(s));
     // Capacity-sized grids: the live count is on the device only.
     const uint32_t n = b->slots;
@@ -1050,10 +1073,18 @@ int kst_run_staged(kst_backend *b, int use_graph) {
         }
         for (uint32_t i = 0; i < b->staged; ++i)
             b->launch_block(b->cfg.max_voices, n, true, true, chan,
-                            b->profile ? &b->stage_ev[6 * i] : nullptr, pipe);
+                            b->profile ? &b->stage_ev[6 * i] : nullptr, pipe, true);
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipStreamSynchronize(s));
+    if (pipe && !use_graph) {
+        HIPCHK(hipStreamSynchronize(b->side));
+        uint32_t timeout = 0;
+        HIPCHK(hipMemcpy(&timeout, &b->st.p->sync_timeout, 4, hipMemcpyDeviceToHost));
+        if (timeout)
+            return fail(KST_EDEVICE, "a device-side wait between the render and selection "
+                                     "streams timed out; the staged results are not valid");
+    }
     if (b->profile && !use_graph) b->read_timings(b->stage_ev.data(), b->staged);
     // Bring the host mirror up to date from the last block's result row.
     BlockResult r;

@@ -129,6 +129,13 @@ struct DevState {
     // victims (SEL_PENDING until the candidates are counted), of a block
     // with sel_want note-ons.
     uint32_t sel_n, sel_parity, sel_steal, sel_want;
+    // Staged runs order the two streams on the device instead of through
+    // events: the first mixdown kernel after block b's render stores b's
+    // sequence number in render_seq (the render has ended, so the pool it
+    // wrote is visible), the selection waits for it; the selection's last
+    // kernel stores it in sel_seq, k_end waits for it. sync_timeout: a wait
+    // gave up (the host reports an error).
+    uint32_t render_seq, sel_seq, sync_timeout, sync_pad;
 };
 
 // Per-block result row, what stats() reads after each render.
@@ -207,7 +214,37 @@ struct Args {
     uint32_t *batch_victim;  // [cap / 64 + 1]
     uint32_t *render_order;  // [XCDS][order_stride]
     uint32_t order_stride;
+    // Device-side ordering of a staged run's two streams (0: none): the
+    // sequence number the mixdown publishes after the render and the
+    // selection waits for, and the one the selection publishes and k_end
+    // waits for.
+    uint32_t sig_render, wait_render, sig_sel, wait_sel;
 };
+
+// A wait gives up after this many ticks of the 100 MHz real-time counter
+// (20 s), records sync_timeout and lets the chain run on, so no kernel can
+// spin forever; every later wait then returns at once.
+constexpr uint64_t SYNC_TIMEOUT_TICKS = 2000000000ull;
+
+// Spin (one thread) until *flag has reached `want` (wrap-safe), with acquire
+// ordering at device scope: what was written before the flag's release is
+// visible afterwards.
+__device__ __forceinline__ void wait_seq(DevState *st, uint32_t *flag, uint32_t want) {
+    if (__hip_atomic_load(&st->sync_timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    const uint64_t t0 = wall_clock64();
+    while ((int32_t)(__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) - want) < 0) {
+        if (wall_clock64() - t0 > SYNC_TIMEOUT_TICKS) {
+            __hip_atomic_store(&st->sync_timeout, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+}
+
+// Publish `seq` in *flag with release ordering at device scope.
+__device__ __forceinline__ void signal_seq(uint32_t *flag, uint32_t seq) {
+    __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // One of a pair of buffers, chosen without indexing the kernel-argument array
 // (a dynamic index would force the pair out to scratch memory).

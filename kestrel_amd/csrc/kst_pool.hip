@@ -140,7 +140,12 @@ __device__ __forceinline__ void sel_begin_body(const Args &a) {
     st->sel_count = n;
     st->sel_done = n <= SEL_CAP;  // few enough for the final workgroup alone
 }
-__global__ void k_sel_begin(Args a) { sel_begin_body(a); }
+// In a staged run the selection of the block after block b first waits until
+// b's render has ended (its mixdown has started).
+__global__ void k_sel_begin(Args a) {
+    if (a.wait_render) wait_seq(a.st, &a.st->render_seq, a.wait_render);
+    sel_begin_body(a);
+}
 __global__ void kb_sel_begin(const Args *__restrict__ aa) { sel_begin_body(aa[blockIdx.z]); }
 
 __device__ __forceinline__ bool sounding_at(const Args &a, const uint32_t *pool, uint32_t i) {
@@ -643,7 +648,15 @@ __device__ __forceinline__ void sel_final_body(const Args &a) {
     }
     if (threadIdx.x == 0 && n == 1 && st->sel_k == 1) mark_victim(a, pool, a.cand_slot[0]);
 }
-__global__ void __launch_bounds__(SEL_THREADS) k_sel_final(Args a) { sel_final_body(a); }
+// The selection's last kernel: in a staged run it then publishes that the
+// victims are marked (every thread's stores are in before thread 0's release).
+__global__ void __launch_bounds__(SEL_THREADS) k_sel_final(Args a) {
+    sel_final_body(a);
+    if (a.sig_sel) {
+        __syncthreads();
+        if (threadIdx.x == 0) signal_seq(&a.st->sel_seq, a.sig_sel);
+    }
+}
 // The same pass over a batch of renders: blockIdx.z picks the render.
 __global__ void __launch_bounds__(SEL_THREADS) kb_sel_final(const Args *__restrict__ aa) {
     sel_final_body(aa[blockIdx.z]);
@@ -766,22 +779,29 @@ __global__ void __launch_bounds__(SORT_TPB) kb_sort_hist(const Args *__restrict_
     sort_hist_body(aa[blockIdx.z], p);
 }
 
-// Exclusive scan of `v` (one value per thread of a 256-thread workgroup) in
-// `buf`; returns the exclusive prefix and leaves the total in *total.
-__device__ __forceinline__ uint32_t block_exscan256(uint32_t v, uint32_t *buf, uint32_t *total) {
-    const uint32_t t = threadIdx.x;
-    buf[t] = v;
-    __syncthreads();
-    for (uint32_t off = 1; off < 256; off <<= 1) {
-        const uint32_t x = t >= off ? buf[t - off] : 0u;
-        __syncthreads();
-        buf[t] += x;
-        __syncthreads();
+// Exclusive scan of `v` (one value per thread of a 256-thread workgroup):
+// each wave scans its 64 values with lane shuffles, the four wave totals meet
+// in `wtot` behind one barrier. Returns the exclusive prefix and leaves the
+// total in *total. `wtot` may be reused only after another barrier.
+__device__ __forceinline__ uint32_t block_exscan256(uint32_t v, uint32_t *wtot, uint32_t *total) {
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t incl = v;
+#pragma unroll
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(incl, off);
+        if (lane >= off) incl += y;
     }
-    const uint32_t incl = buf[t];
-    *total = buf[255];
+    if (lane == 63) wtot[wave] = incl;
     __syncthreads();
-    return incl - v;
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < SORT_TPB / 64; ++w) {
+        const uint32_t x = wtot[w];
+        before += w < wave ? x : 0u;
+        all += x;
+    }
+    *total = all;
+    return before + incl - v;
 }
 
 // Each tile's offset within a digit is the same digit's count in the tiles
@@ -829,14 +849,16 @@ __global__ void __launch_bounds__(1024) kb_sort_scan(const Args *__restrict__ aa
     sort_scan_body(aa[blockIdx.z], p);
 }
 
-// Stable scatter of one digit pass. Items of a tile are ranked in index
-// order: rounds of 256 consecutive items, waves in order within a round,
-// lanes in order within a wave (match masks built from ballots). A digit's
-// first lane in each wave publishes the wave's count for that digit and
-// clears it again after the round, so the per-wave table stays zero between
-// rounds without a full sweep. Ranked items are first placed in LDS in digit
-// order (a stable local sort of the tile), then written out by consecutive
-// threads, so each digit's run within the tile leaves in adjacent addresses.
+// Stable scatter of one digit pass. Each wave ranks its own contiguous quarter
+// of the tile (512 items, 8 rounds of 64 in index order) against a per-wave
+// digit table, with match masks built from ballots and no workgroup barrier
+// between rounds. The four tables then become per-wave offsets within each
+// digit, the tile's digit counts are scanned into local bases, the ranked tile
+// is placed in LDS in digit order (a stable local sort) and written out by
+// consecutive threads, so each digit's run within the tile leaves in adjacent
+// addresses. Each XCD takes a contiguous range of tiles: the runs of one digit
+// from neighbouring tiles are adjacent in the output, so their cache lines
+// fill within one L2.
 __device__ __forceinline__ void sort_scatter_body(const Args &a, uint32_t p) {
     if (p >= a.sort_passes) return;  // a batch runs the most passes any member needs
     DevState *st = a.st;
@@ -849,17 +871,19 @@ __device__ __forceinline__ void sort_scatter_body(const Args &a, uint32_t p) {
     uint32_t *kout = sel2(a.keys, (p + 1) & 1);
     uint32_t *iout = sel2(a.idx, (p + 1) & 1);
     constexpr uint32_t WAVES = SORT_TPB / 64;
-    constexpr uint32_t PER = SORT_MAX_RADIX / SORT_TPB;
-    __shared__ uint32_t base[SORT_MAX_RADIX];
-    __shared__ uint32_t cnt[SORT_MAX_RADIX];
-    __shared__ uint32_t wc[WAVES][SORT_MAX_RADIX];
-    __shared__ uint32_t buf[256];
+    constexpr uint32_t PER = SORT_MAX_RADIX / SORT_TPB;  // digits per thread
+    constexpr uint32_t CHUNK = SORT_TILE / WAVES;        // items per wave
+    static_assert(CHUNK / 64 == SORT_IPT, "a wave ranks SORT_IPT rounds of 64 items");
+    __shared__ uint32_t wcnt[WAVES][SORT_MAX_RADIX];  // per-wave counts, then offsets
+    __shared__ uint32_t wtot[WAVES];
     __shared__ uint32_t gbase[SORT_MAX_RADIX];  // the tile's first output slot per digit
     __shared__ uint32_t lbase[SORT_MAX_RADIX];  // the digit's first slot in the local order
     __shared__ uint32_t lkey[SORT_TILE], lidx[SORT_TILE];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-    // Digit bases: exclusive scan of this pass's digit totals.
+    // Digit bases: exclusive scan of this pass's digit totals; thread i holds
+    // digits i * PER .. i * PER + PER - 1.
+    uint32_t dbase[PER];
     {
         uint32_t v[PER], sum = 0;
 #pragma unroll
@@ -869,66 +893,52 @@ __device__ __forceinline__ void sort_scatter_body(const Args &a, uint32_t p) {
             sum += v[j];
         }
         uint32_t total;
-        uint32_t run = block_exscan256(sum, buf, &total);
+        uint32_t run = block_exscan256(sum, wtot, &total);
 #pragma unroll
         for (uint32_t j = 0; j < PER; ++j) {
-            base[threadIdx.x * PER + j] = run;
+            dbase[j] = run;
             run += v[j];
         }
     }
-    for (uint32_t i = threadIdx.x; i < WAVES * SORT_MAX_RADIX; i += SORT_TPB)
-        wc[i / SORT_MAX_RADIX][i % SORT_MAX_RADIX] = 0;
-    __syncthreads();
     // Fused: survivors sort before every dead key, so when the dead keys begin
     // at a digit boundary of the last pass, their count is that digit's base.
-    if (a.fused && p + 1 == a.sort_passes && a.dead_digit != ~0u && blockIdx.x == 0 &&
-        threadIdx.x == 0)
-        st->live_new = a.dead_digit < radix ? base[a.dead_digit] : n;
-    for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
-        uint32_t kk[SORT_IPT], ii[SORT_IPT];
+    if (a.fused && p + 1 == a.sort_passes && a.dead_digit != ~0u && blockIdx.x == 0) {
+        if (a.dead_digit >= radix) {
+            if (threadIdx.x == 0) st->live_new = n;
+        } else if (a.dead_digit / PER == threadIdx.x) {
+#pragma unroll
+            for (uint32_t j = 0; j < PER; ++j)
+                if (a.dead_digit % PER == j) st->live_new = dbase[j];
+        }
+    }
+    for (uint32_t id = blockIdx.x; id < tiles; id += gridDim.x) {
+        const uint32_t t = xcd_group(id, tiles);
+        const uint32_t t0 = t * SORT_TILE;
+        __syncthreads();  // the previous tile's readers of the LDS tables are done
+        uint32_t kk[SORT_IPT], ii[SORT_IPT], rk[SORT_IPT], hv[PER];
+        const uint32_t i0 = t0 + wave * CHUNK + lane;
 #pragma unroll
         for (uint32_t k = 0; k < SORT_IPT; ++k) {
-            const uint32_t item = t * SORT_TILE + k * SORT_TPB + threadIdx.x;
+            const uint32_t item = i0 + k * 64;
             const bool valid = item < n;
             kk[k] = valid ? kin[item] : 0u;
             ii[k] = valid ? (p == 0 ? item : iin[item]) : 0u;
         }
-        for (uint32_t d = threadIdx.x; d < SORT_MAX_RADIX; d += SORT_TPB) {
-            const uint32_t g = d < radix ? base[d] + a.sort_hist[((size_t)p * a.tiles_max + t) * radix + d] : 0u;
-            cnt[d] = g;
-            gbase[d] = g;
-            lbase[d] = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < PER; ++j) {
+            const uint32_t d = threadIdx.x * PER + j;
+            hv[j] = d < radix ? a.sort_hist[((size_t)p * a.tiles_max + t) * radix + d] : 0u;
         }
-        __syncthreads();
-        // The tile's own digit counts, then their exclusive scan: lbase.
+        for (uint32_t d = lane; d < SORT_MAX_RADIX; d += 64) wcnt[wave][d] = 0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // Ranks within the wave's quarter: the lanes of one digit read its
+        // running count, the lowest of them advances it by their number.
 #pragma unroll
         for (uint32_t k = 0; k < SORT_IPT; ++k) {
-            const uint32_t item = t * SORT_TILE + k * SORT_TPB + threadIdx.x;
-            if (item < n) atomicAdd(&lbase[(kk[k] >> shift) & mask], 1u);
-        }
-        __syncthreads();
-        {
-            uint32_t v[PER], sum = 0;
-#pragma unroll
-            for (uint32_t j = 0; j < PER; ++j) {
-                v[j] = lbase[threadIdx.x * PER + j];
-                sum += v[j];
-            }
-            uint32_t total;
-            uint32_t run = block_exscan256(sum, buf, &total);
-#pragma unroll
-            for (uint32_t j = 0; j < PER; ++j) {
-                lbase[threadIdx.x * PER + j] = run;
-                run += v[j];
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (uint32_t k = 0; k < SORT_IPT; ++k) {
-            const uint32_t item = t * SORT_TILE + k * SORT_TPB + threadIdx.x;
-            const bool valid = item < n;
-            const uint32_t key = kk[k];
-            const uint32_t digit = (key >> shift) & mask;
+            const bool valid = i0 + k * 64 < n;
+            const uint32_t digit = (kk[k] >> shift) & mask;
             uint64_t m = __ballot(valid);
             if (a.digit_bits == SORT_MAX_BITS) {  // the usual width, unrolled
 #pragma unroll
@@ -942,28 +952,52 @@ __device__ __forceinline__ void sort_scatter_body(const Args &a, uint32_t p) {
                     m &= ((digit >> bit) & 1) ? bb : ~bb;
                 }
             }
-            const uint32_t rank = __popcll(m & lt);
-            const bool leader = valid && rank == 0;
-            if (leader) wc[wave][digit] = __popcll(m);
-            __syncthreads();
-            if (valid) {
-                uint32_t off = cnt[digit] + rank;
+            const uint32_t below = __popcll(m & lt);
+            const uint32_t pre = valid ? wcnt[wave][digit] : 0u;
+            rk[k] = pre + below;
+            if (valid && below == 0) wcnt[wave][digit] = pre + (uint32_t)__popcll(m);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        __syncthreads();
+        // Per digit: each wave's offset (the earlier waves' counts), the
+        // tile's count, and its local and global bases.
+        uint32_t c[PER], sum = 0;
 #pragma unroll
-                for (uint32_t w = 0; w < WAVES - 1; ++w)
-                    if (w < wave) off += wc[w][digit];
-                const uint32_t lpos = lbase[digit] + (off - gbase[digit]);
-                lkey[lpos] = key;
-                lidx[lpos] = ii[k];
+        for (uint32_t j = 0; j < PER; ++j) {
+            const uint32_t d = threadIdx.x * PER + j;
+            uint32_t run = 0;
+#pragma unroll
+            for (uint32_t w = 0; w < WAVES; ++w) {
+                const uint32_t x = wcnt[w][d];
+                wcnt[w][d] = run;
+                run += x;
             }
-            __syncthreads();
-            if (leader) {
-                atomicAdd(&cnt[digit], (uint32_t)__popcll(m));
-                wc[wave][digit] = 0;
+            c[j] = run;
+            sum += run;
+        }
+        uint32_t total;
+        uint32_t run = block_exscan256(sum, wtot, &total);
+#pragma unroll
+        for (uint32_t j = 0; j < PER; ++j) {
+            const uint32_t d = threadIdx.x * PER + j;
+            lbase[d] = run;
+            gbase[d] = dbase[j] + hv[j];
+            run += c[j];
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t k = 0; k < SORT_IPT; ++k) {
+            if (i0 + k * 64 < n) {
+                const uint32_t d = (kk[k] >> shift) & mask;
+                const uint32_t lpos = lbase[d] + wcnt[wave][d] + rk[k];
+                lkey[lpos] = kk[k];
+                lidx[lpos] = ii[k];
             }
         }
         __syncthreads();
         // Out in local order: slot j holds the (j - lbase[d])-th item of digit d.
-        const uint32_t t0 = t * SORT_TILE;
         const uint32_t items = n - t0 < SORT_TILE ? n - t0 : SORT_TILE;
         for (uint32_t j = threadIdx.x; j < items; j += SORT_TPB) {
             const uint32_t key = lkey[j];
@@ -972,7 +1006,6 @@ __device__ __forceinline__ void sort_scatter_body(const Args &a, uint32_t p) {
             kout[off] = key;
             iout[off] = lidx[j];
         }
-        __syncthreads();
     }
 }
 __global__ void __launch_bounds__(SORT_TPB) k_sort_scatter(Args a, uint32_t p) { sort_scatter_body(a, p); }
@@ -1072,7 +1105,14 @@ __device__ __forceinline__ void end_body(const Args &a) {
     // keeps this block's counts for a settle's gather).
     if (a.end_begins && st->block < a.nblocks) begin_body(a);  // thread 0, as begin_body needs
 }
-__global__ void k_end(Args a) { end_body(a); }
+// In a staged run the block ends once the next block's victims are marked.
+__global__ void k_end(Args a) {
+    if (a.wait_sel) {
+        if (threadIdx.x == 0) wait_seq(a.st, &a.st->sel_seq, a.wait_sel);
+        __syncthreads();
+    }
+    end_body(a);
+}
 
 // The per-block render's dispatch lists (one workgroup per XCD). The live
 // pool's survivor batches [0, nsurv) are split into XCDS contiguous runs, as

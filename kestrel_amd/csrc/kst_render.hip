@@ -1115,6 +1115,9 @@ kb_render_fast(const Args *__restrict__ aa) {
 template <bool COMP>
 __device__ __forceinline__ void reduce_rows_body(const Args &a) {
     DevState *st = a.st;
+    // The render has ended: publish it to the selection on the other stream.
+    if (a.sig_render && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0)
+        signal_seq(&st->render_seq, a.sig_render);
     const uint32_t two_f = 2 * a.F;
     const uint32_t rows = (st->live_total + 63) / 64;
     const uint32_t chunk = blockIdx.y;
