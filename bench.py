@@ -136,12 +136,14 @@ WORKLOADS = {
 
 
 def timed_profile(args) -> int:
-    """The timed run records HIP events around the render kernel only (its
-    duration is the roofline's denominator); events between every pass would
-    add a few microseconds per pass to the block. The per-pass breakdown comes
-    from a replay of the same blocks afterwards."""
-    from kestrel_amd.config import PROFILE_OFF, PROFILE_RENDER
-    return PROFILE_OFF if args.graph else PROFILE_RENDER
+    """The timed run records HIP events around the render kernel of every
+    RENDER_SAMPLE-th block (blocks 0, 8, 16, ...): their mean duration is the
+    roofline's denominator. Each event packet costs the chain a few
+    microseconds, so the other blocks run with none; events between every pass
+    would add them to every pass. The per-pass breakdown comes from a replay of
+    the same blocks afterwards."""
+    from kestrel_amd.config import PROFILE_OFF, PROFILE_RENDER_SAMPLED
+    return PROFILE_OFF if args.graph else PROFILE_RENDER_SAMPLED
 
 
 def setup(name: str, args, seconds: float):
@@ -772,7 +774,7 @@ def main():
     # with events between every pass (outside the timed region) ----
     passes = None
     if not args.graph:
-        from kestrel_amd.config import PROFILE_ALL, PROFILE_RENDER
+        from kestrel_amd.config import PROFILE_ALL
         nb_b = min(K, 64)
         gpu.pool_import(snapshot)
         gpu.set_counters(st0["stolen"], st0["dropped"])
@@ -780,7 +782,7 @@ def main():
         gpu.set_profile(PROFILE_ALL)
         gpu.run_staged(False)
         passes = {k: round(v, 4) for k, v in gpu.timings()}
-        gpu.set_profile(PROFILE_RENDER)
+        gpu.set_profile(timed_profile(args))
         log(f"per-pass ms/block (replay of {nb_b} timed blocks with events between passes): "
             f"{passes}")
 
@@ -792,6 +794,7 @@ def main():
     bytes_block = V * (192.0 + 2.0 * (taps - 1) + 2.0 * F * mean_step) + 76.0 * spawned + 8.0 * F
     per_unit = float(bytes_block.sum() / voice_samples)
     render_ms = timings.get("render")
+    from kestrel_amd.config import PROFILE_RENDER_SAMPLED, RENDER_SAMPLE
     roofline = None
     if render_ms:
         achieved = float(bytes_block.mean()) / (render_ms * 1e-3) / 1e9
@@ -801,7 +804,10 @@ def main():
                     "traffic": traffic, "traffic_source": source, "kernel": "k_render_fast",
                     "algorithmic_bytes_per_launch": round(float(bytes_block.mean())),
                     "bytes_per_voice_sample": round(per_unit, 4),
-                    "render_ms_per_block": round(render_ms, 4)}
+                    "render_ms_per_block": round(render_ms, 4),
+                    "render_timed_blocks": (len(range(0, K, RENDER_SAMPLE))
+                                            if timed_profile(args) == PROFILE_RENDER_SAMPLED
+                                            else K)}
     log(f"roofline: {roofline}")
 
     # ---- CPU baseline + parity over consecutive timed blocks ----

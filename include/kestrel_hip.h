@@ -33,6 +33,9 @@ extern "C" {
 #define KST_PROFILE_OFF 0u
 #define KST_PROFILE_ALL 1u     /* events between every pass of every block */
 #define KST_PROFILE_RENDER 2u  /* events around the render kernel only */
+#define KST_PROFILE_RENDER_SAMPLED 3u  /* as _RENDER, but in a staged run only around the
+                                          render of every KST_RENDER_SAMPLE-th block */
+#define KST_RENDER_SAMPLE 8u
 
 /* One voice's birth record. src/voice.rs:22-57, 76 bytes, same field order. */
 typedef struct kst_spawn_cmd {
@@ -95,7 +98,8 @@ typedef struct kst_config {
     uint32_t sort_digit_bits;       /* radix width of the locality sort, 1..8 */
     int32_t device;                 /* HIP device ordinal */
     uint32_t profile;               /* per-pass timings (src/config.rs:337-338):
-                                       KST_PROFILE_OFF, _ALL or _RENDER */
+                                       KST_PROFILE_OFF, _ALL, _RENDER or
+                                       _RENDER_SAMPLED */
 } kst_config;
 
 typedef struct kst_backend kst_backend;
@@ -158,7 +162,10 @@ size_t kst_timings(const kst_backend *b, kst_timing *out, size_t cap);
 /* Change the profile level of a backend between blocks (KST_PROFILE_OFF,
  * KST_PROFILE_ALL: events between every pass, KST_PROFILE_RENDER: events
  * around the render kernel only, which leaves the rest of the chain as
- * without profiling). No counterpart in the reference: the config's
+ * without profiling, KST_PROFILE_RENDER_SAMPLED: the same on every
+ * KST_RENDER_SAMPLE-th block of a staged run, blocks 0, 8, 16, ..., so that
+ * the other blocks run with no event packets at all; the timing is the mean
+ * over the sampled blocks). No counterpart in the reference: the config's
  * `profile` fixes it for the backend's life there (src/config.rs:337-338). */
 int kst_set_profile(kst_backend *b, uint32_t level);
 

@@ -102,8 +102,9 @@ class HipSynth:
         return self._lib.kst_name(self._h).decode()
 
     def set_profile(self, level: int) -> None:
-        """PROFILE_OFF, PROFILE_ALL (events between every pass) or
-        PROFILE_RENDER (events around the render kernel only)."""
+        """PROFILE_OFF, PROFILE_ALL (events between every pass), PROFILE_RENDER
+        (events around the render kernel only) or PROFILE_RENDER_SAMPLED (the
+        same around every RENDER_SAMPLE-th block of a staged run)."""
         abi.check(self._lib.kst_set_profile(self._h, int(level)))
 
     def timings(self) -> list[tuple[str, float]]:

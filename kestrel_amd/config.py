@@ -18,8 +18,10 @@ ADMIT_LOUDEST, ADMIT_EVEN = 0, 1
 
 FORM_PLAIN, FORM_WAVE64 = 0, 1
 # kst_config.profile: off, HIP events between every pass, or around the render
-# kernel only (the rest of the block chain runs as unprofiled).
-PROFILE_OFF, PROFILE_ALL, PROFILE_RENDER = 0, 1, 2
+# kernel only (the rest of the block chain runs as unprofiled), or that on every
+# RENDER_SAMPLE-th block of a staged run (the others run with no events).
+PROFILE_OFF, PROFILE_ALL, PROFILE_RENDER, PROFILE_RENDER_SAMPLED = 0, 1, 2, 3
+RENDER_SAMPLE = 8  # KST_RENDER_SAMPLE: PROFILE_RENDER_SAMPLED times blocks 0, 8, 16, ...
 
 
 class ConfigError(ValueError):
@@ -69,7 +71,7 @@ class Config:
     waves_per_group: int = 1
     sort_digit_bits: int = 9
     device: int = 0
-    profile: int = 0  # PROFILE_OFF, PROFILE_ALL or PROFILE_RENDER
+    profile: int = 0  # PROFILE_OFF, PROFILE_ALL, PROFILE_RENDER or PROFILE_RENDER_SAMPLED
 
     def validate(self) -> None:
         """src/config.rs:412-525, the render-path subset."""

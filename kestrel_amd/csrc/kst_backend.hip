@@ -450,7 +450,7 @@ struct kst_backend {
         // render kernel: each event in the stream costs the chain a few
         // microseconds.
         const bool timed_render = ev != nullptr;
-        const bool timed = timed_render && profile != KST_PROFILE_RENDER;
+        const bool timed = timed_render && !render_only();
         const uint32_t rg = render_groups_for(n_total);
         Args a = args(rg);
         hipStream_t s = stream;
@@ -502,19 +502,26 @@ struct kst_backend {
         if (timed) (void)hipEventRecord(ev[5], s);
     }
 
-    // Per-pass milliseconds, averaged over `nblocks` event sets of 6 (only
-    // the render's pair under KST_PROFILE_RENDER).
-    void read_timings(const hipEvent_t *evs, uint32_t nblocks) {
+    bool render_only() const {
+        return profile == KST_PROFILE_RENDER || profile == KST_PROFILE_RENDER_SAMPLED;
+    }
+
+    // Per-pass milliseconds, averaged over the event sets of 6 of blocks 0,
+    // stride, 2 * stride, ... below `nblocks` (only the render's pair under
+    // the render-only levels).
+    void read_timings(const hipEvent_t *evs, uint32_t nblocks, uint32_t stride = 1) {
         timings.clear();
         for (int i = 0; i < 5; ++i) {
-            if (profile == KST_PROFILE_RENDER && i != 2) continue;
+            if (render_only() && i != 2) continue;
             double sum = 0.0;
-            for (uint32_t k = 0; k < nblocks; ++k) {
+            uint32_t cnt = 0;
+            for (uint32_t k = 0; k < nblocks; k += stride) {
                 float ms = 0.0f;
                 if (hipEventElapsedTime(&ms, evs[6 * k + i], evs[6 * k + i + 1]) == hipSuccess)
                     sum += ms;
+                ++cnt;
             }
-            timings.push_back({PASS_NAMES[i], sum / nblocks});
+            timings.push_back({PASS_NAMES[i], cnt ? sum / cnt : 0.0});
         }
     }
 
@@ -755,7 +762,7 @@ int kst_create(const kst_config *cfg, const int16_t *pool, size_t pool_len,
     for (auto &x : b->ev)
         if (hipEventCreate(&x) != hipSuccess)
             return cleanup(fail(KST_EDEVICE, "creating timing events"));
-    b->profile = c.profile <= KST_PROFILE_RENDER ? c.profile : KST_PROFILE_ALL;
+    b->profile = c.profile <= KST_PROFILE_RENDER_SAMPLED ? c.profile : KST_PROFILE_ALL;
     if (b->profile == 0 && getenv("KST_PROFILE") != nullptr) b->profile = KST_PROFILE_ALL;
     *out = b;
     return KST_OK;
@@ -979,9 +986,10 @@ const char *kst_name(const kst_backend *b) {
 
 int kst_set_profile(kst_backend *b, uint32_t level) {
     if (!b) return fail(KST_EINVAL, "null backend");
-    if (level > KST_PROFILE_RENDER)
-        return fail(KST_EINVAL, "profile level %u must be 0, 1 (every pass) or 2 (render only)",
-                    level);
+    if (level > KST_PROFILE_RENDER_SAMPLED)
+        return fail(KST_EINVAL,
+                    "profile level %u must be 0, 1 (every pass), 2 (render only) or 3 (render "
+                    "only, every %u-th staged block)", level, KST_RENDER_SAMPLE);
     if (b->inflight) return fail(KST_ESTATE, "a render is in flight");
     b->profile = level;
     b->cfg.profile = level;
@@ -1071,9 +1079,11 @@ int kst_run_staged(kst_backend *b, int use_graph) {
             int rc = b->ensure_stage_events(b->staged);
             if (rc) return rc;
         }
+        const uint32_t stride = b->profile == KST_PROFILE_RENDER_SAMPLED ? KST_RENDER_SAMPLE : 1;
         for (uint32_t i = 0; i < b->staged; ++i)
             b->launch_block(b->cfg.max_voices, n, true, true, chan,
-                            b->profile ? &b->stage_ev[6 * i] : nullptr, pipe, true);
+                            b->profile && i % stride == 0 ? &b->stage_ev[6 * i] : nullptr, pipe,
+                            true);
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipStreamSynchronize(s));
@@ -1085,7 +1095,9 @@ int kst_run_staged(kst_backend *b, int use_graph) {
             return fail(KST_EDEVICE, "a device-side wait between the render and selection "
                                      "streams timed out; the staged results are not valid");
     }
-    if (b->profile && !use_graph) b->read_timings(b->stage_ev.data(), b->staged);
+    if (b->profile && !use_graph)
+        b->read_timings(b->stage_ev.data(), b->staged,
+                        b->profile == KST_PROFILE_RENDER_SAMPLED ? KST_RENDER_SAMPLE : 1);
     // Bring the host mirror up to date from the last block's result row.
     BlockResult r;
     HIPCHK(hipMemcpy(&r, b->g_results.p + (b->staged - 1), sizeof r, hipMemcpyDeviceToHost));

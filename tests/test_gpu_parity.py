@@ -322,14 +322,16 @@ def test_validation_errors_name_the_flag():
 
 
 def test_profile_levels_time_the_passes_they_name():
-    """PROFILE_RENDER times the render kernel only, PROFILE_ALL every pass;
-    either way the staged blocks' outputs are the unprofiled ones."""
+    """PROFILE_RENDER times the render kernel only, PROFILE_RENDER_SAMPLED the
+    same on every 8th staged block, PROFILE_ALL every pass; either way the
+    staged blocks' outputs are the unprofiled ones."""
     from kestrel_amd.abi import BackendError
     from kestrel_amd.backend import HipSynth
-    from kestrel_amd.config import PROFILE_ALL, PROFILE_OFF, PROFILE_RENDER
+    from kestrel_amd.config import (PROFILE_ALL, PROFILE_OFF, PROFILE_RENDER,
+                                    PROFILE_RENDER_SAMPLED)
     cfg, bank, notes = W.workload_c1()
     outs = {}
-    for level in (PROFILE_OFF, PROFILE_RENDER, PROFILE_ALL):
+    for level in (PROFILE_OFF, PROFILE_RENDER, PROFILE_RENDER_SAMPLED, PROFILE_ALL):
         g = HipSynth(cfg.replace(profile=level), bank.pool, bank.params, bank.n_regions,
                      bank.max_sample_len)
         bb = W.BlockBuilder(cfg, bank, notes)
@@ -343,13 +345,14 @@ def test_profile_levels_time_the_passes_they_name():
         outs[level], _ = g.staged_results()
         names = [n for n, _ in g.timings()]
         assert names == {PROFILE_OFF: [], PROFILE_RENDER: ["render"],
+                         PROFILE_RENDER_SAMPLED: ["render"],
                          PROFILE_ALL: ["steal", "spawn", "render", "reduce", "compact"]}[level]
         assert all(ms > 0 for _, ms in g.timings())
         with pytest.raises(BackendError, match="profile level"):
-            g.set_profile(3)
+            g.set_profile(4)
         g.close()
-    assert np.array_equal(outs[PROFILE_OFF], outs[PROFILE_RENDER])
-    assert np.array_equal(outs[PROFILE_OFF], outs[PROFILE_ALL])
+    for level in (PROFILE_RENDER, PROFILE_RENDER_SAMPLED, PROFILE_ALL):
+        assert np.array_equal(outs[PROFILE_OFF], outs[level])
 
 
 @pytest.mark.parametrize("rule,digit_bits", [(2, 8), (0, 9), (2, 5)])
