@@ -171,7 +171,9 @@ constexpr uint32_t SEL_UNROLL = 4;
 // ---------------------------------------------------------------------------
 // Steal selection: the steal_k smallest keys, which are unique (the note id is
 // in every key), so "key <= the k-th smallest" names exactly steal_k voices --
-// the set src/cpu.rs:351-370 takes with select_nth_unstable.
+// the set src/cpu.rs:351-370 takes with select_nth_unstable. Should a caller
+// repeat note ids, every voice whose key is at or below the k-th smallest is a
+// victim (the oracle's rule), so a group tied at the threshold goes whole.
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long x) {
@@ -508,8 +510,10 @@ __device__ void sel_final_regs(const Args &a, uint32_t n) {
         key[j] = in ? (uint32_t)a.cand_key[i] : 0u;
     }
     uint32_t top = st->sel_top, k = st->sel_k, count = n;
-    // Each round lowers top by at least SEL_BITS; unique keys leave one.
-    for (uint32_t round = 0; round <= 32 / SEL_BITS && count > 1; ++round) {
+    // Each round lowers top by at least SEL_BITS; unique keys leave one, keys
+    // that tie leave their group (same: every active key is the same value).
+    bool same = top == 0;
+    for (uint32_t round = 0; round <= 32 / SEL_BITS && count > 1 && !same; ++round) {
         const uint32_t lo = sel_lo(top);
         const uint32_t dmask = (1u << (top - lo)) - 1;
         if (t < SEL_BINS) {
@@ -563,12 +567,15 @@ __device__ void sel_final_regs(const Args &a, uint32_t n) {
             if (d != bin) act &= ~(1u << j);
         }
         top = diff ? 32 - __clz(diff) : 0;
+        same = diff == 0;
         __syncthreads();  // every read of this round's bins before the next clears them
     }
-    // The one key left is the k-th smallest.
+    // The keys left are the k-th smallest: one, or a group of equal keys (note
+    // ids the caller repeated), every one of which is at the threshold and a
+    // victim, as every key at or below the threshold is in the oracle.
 #pragma unroll
     for (uint32_t j = 0; j < PER; ++j)
-        if (((act >> j) & 1u) && count == 1 && k == 1)
+        if (((act >> j) & 1u) && (count == 1 || same))
             mark_victim(a, pool, a.cand_slot[j * SEL_THREADS + t]);
 }
 
@@ -599,8 +606,10 @@ __device__ __forceinline__ void sel_final_body(const Args &a) {
     uint32_t n = st->cand_n;
     // Each round lowers sel_top by at least SEL_BITS (the chosen bin's keys
     // agree on every bit above the digit), and unique keys leave one
-    // candidate when it reaches 0; the bound only guards a broken state.
-    for (uint32_t round = 0; round <= 64 / SEL_BITS && st->sel_count > 1; ++round) {
+    // candidate when it reaches 0 (equal keys leave their group); the bound
+    // only guards a broken state.
+    for (uint32_t round = 0; round <= 64 / SEL_BITS && st->sel_count > 1 && st->sel_top > 0;
+         ++round) {
         const uint32_t top = st->sel_top;
         const uint32_t lo = sel_lo(top);
         const uint32_t dmask = (1u << (top - lo)) - 1;
@@ -646,7 +655,10 @@ __device__ __forceinline__ void sel_final_body(const Args &a) {
         n = s_n;
         __syncthreads();
     }
-    if (threadIdx.x == 0 && n == 1 && st->sel_k == 1) mark_victim(a, pool, a.cand_slot[0]);
+    // The keys left are the k-th smallest: one, or a group of equal keys, all
+    // at the threshold and victims (every key at or below it is one).
+    if (st->sel_count <= 1 || st->sel_top == 0)
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) mark_victim(a, pool, a.cand_slot[i]);
 }
 // The selection's last kernel: in a staged run it then publishes that the
 // victims are marked (every thread's stores are in before thread 0's release).

@@ -233,6 +233,49 @@ def test_pipelined_staged_run_equals_per_block_calls_while_stealing():
     a.close()
 
 
+def test_repeated_note_ids_steal_every_voice_at_the_threshold():
+    """Every block re-spawns the same notes with the same note ids, so steal
+    keys tie: the oracle marks every voice whose key is at or below the
+    steal_k-th smallest, and so must the device, per block and in a staged run
+    whose selection runs ahead on the side stream (pool bit-exact, counters,
+    null)."""
+    cfg = W.config_c2(max_voices=40_000)
+    bank = W.make_bank(11, instruments=16, zones=4, key_lo=0, key_hi=127)
+    notes = W.burst_notes(12, 0.6, base_rate_hz=150_000.0, burst_notes_n=20_000,
+                          burst_every_s=0.25)
+    # A fresh builder per block: block 0's notes (ids from 0) every time.
+    blocks = [W.BlockBuilder(cfg, bank, notes).next_block() for _ in range(30)]
+    print("repeated note ids, 40k voices, 30 blocks", flush=True)
+    gpu, cpu = make_pair(cfg, bank)
+    worst = -math.inf
+    for b, blk in enumerate(blocks):
+        og, oc = step_both(gpu, cpu, blk)
+        worst = max(worst, null_db(og, oc))
+        compare_pools(gpu, cpu, f"block {b}")
+    assert gpu.stats()["stolen"] > 0
+    assert worst <= NULL_DB, worst
+    gpu.close()
+    from kestrel_amd.backend import HipSynth
+    from oracle import oracle
+    g = HipSynth(cfg, bank.pool, bank.params, bank.n_regions, bank.max_sample_len)
+    c = oracle.Oracle(cfg, bank.pool, bank.params)
+    c.set_sorted(bank.n_regions, bank.max_sample_len, True)
+    stage_all(g, blocks)
+    g.run_staged(False)
+    outs, stats = g.staged_results()
+    for b, blk in enumerate(blocks):
+        c.set_gates(blk.gates)
+        c.set_channels(blk.chans, *blk.flags)
+        c.spawn(blk.cmds)
+        oc = c.render()
+        sc = c.stats()
+        assert (stats[b]["active_voices"], stats[b]["stolen"], stats[b]["dropped"]) == \
+            (sc["active_voices"], sc["stolen"], sc["dropped"]), b
+        assert null_db(outs[b], oc) <= NULL_DB, b
+    compare_pools(g, c, "staged run")
+    g.close()
+
+
 def stage_all(g, blocks):
     g.stage_blocks(np.concatenate([x.gates for x in blocks]),
                    np.concatenate([x.chans for x in blocks]),
