@@ -427,10 +427,13 @@ __global__ void __launch_bounds__(SEL_THREADS) kb_sel_refine(const Args *__restr
     sel_refine_body(aa[blockIdx.z], blocks);
 }
 
-__device__ __forceinline__ void mark_victim(const Args &a, uint32_t *pool, uint32_t i) {
-    // src/cpu.rs:365-369 / shaders/compact.wgsl:174-175
-    const uint32_t lo = pool[F_NOTE_LO * a.cap + i];
-    pool[F_STOP_REL * a.cap + i] = lo % a.steal_span + 1;
+// Slot i's voice is a victim: its stop frame from the low word of its note id
+// (src/cpu.rs:365-369 / shaders/compact.wgsl:174-175). Under either rule that
+// word is the steal key's low word, so callers pass it from the key rather
+// than loading it again.
+__device__ __forceinline__ void mark_victim(const Args &a, uint32_t *pool, uint32_t i,
+                                            uint32_t note_lo) {
+    pool[F_STOP_REL * a.cap + i] = note_lo % a.steal_span + 1;
 }
 
 // Every key below the candidates is a victim; the candidates are collected
@@ -456,7 +459,7 @@ __device__ __forceinline__ void sel_collect_body(const Args &a) {
         for (uint32_t u = 0; u < SEL_UNROLL; ++u) {
             if (!ok[u]) continue;
             const unsigned long long kb = top >= 64 ? 0ull : key[u] >> top;
-            if (kb < pb) mark_victim(a, pool, i0 + u * stride);
+            if (kb < pb) mark_victim(a, pool, i0 + u * stride, (uint32_t)key[u]);
             else if (kb == pb) atomicAdd(&s_n, 1u);
         }
     }
@@ -563,7 +566,7 @@ __device__ void sel_final_regs(const Args &a, uint32_t n) {
         for (uint32_t j = 0; j < PER; ++j) {
             if (!((act >> j) & 1u)) continue;
             const uint32_t d = (key[j] >> lo) & dmask;
-            if (d < bin) mark_victim(a, pool, a.cand_slot[j * SEL_THREADS + t]);
+            if (d < bin) mark_victim(a, pool, a.cand_slot[j * SEL_THREADS + t], key[j]);
             if (d != bin) act &= ~(1u << j);
         }
         top = diff ? 32 - __clz(diff) : 0;
@@ -576,7 +579,7 @@ __device__ void sel_final_regs(const Args &a, uint32_t n) {
 #pragma unroll
     for (uint32_t j = 0; j < PER; ++j)
         if (((act >> j) & 1u) && (count == 1 || same))
-            mark_victim(a, pool, a.cand_slot[j * SEL_THREADS + t]);
+            mark_victim(a, pool, a.cand_slot[j * SEL_THREADS + t], key[j]);
 }
 
 // Finish on the candidates with more rounds in one workgroup until the bin
@@ -641,7 +644,7 @@ __device__ __forceinline__ void sel_final_body(const Args &a) {
                 k = a.cand_key[i];
                 sl = a.cand_slot[i];
                 const unsigned long long kb = ntop >= 64 ? 0ull : k >> ntop;
-                if (kb < pb) mark_victim(a, pool, sl);
+                if (kb < pb) mark_victim(a, pool, sl, (uint32_t)k);
                 keep = kb == pb;
             }
             __syncthreads();  // every read of this chunk before any write
@@ -658,7 +661,8 @@ __device__ __forceinline__ void sel_final_body(const Args &a) {
     // The keys left are the k-th smallest: one, or a group of equal keys, all
     // at the threshold and victims (every key at or below it is one).
     if (st->sel_count <= 1 || st->sel_top == 0)
-        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) mark_victim(a, pool, a.cand_slot[i]);
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
+            mark_victim(a, pool, a.cand_slot[i], (uint32_t)a.cand_key[i]);
 }
 // The selection's last kernel: in a staged run it then publishes that the
 // victims are marked (every thread's stores are in before thread 0's release).
