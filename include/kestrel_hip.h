@@ -184,7 +184,15 @@ int kst_stage_blocks(kst_backend *b, uint32_t nblocks, const uint32_t *gates,
                      const uint32_t *chans, const uint32_t *chan_flags,
                      const kst_spawn_cmd *cmds, const uint32_t *cmd_counts);
 /* Runs every staged block. use_graph != 0 replays the per-block launch chain
- * from a captured hipGraph. Returns when the device is done. */
+ * from a captured hipGraph. Returns when the device is done.
+ * Pools from 32,768 voices select each block's victims on a second stream
+ * during the block before. Without a graph the two streams are ordered on the
+ * device (bounded spin waits on sequence flags), which needs kernels of the two
+ * streams to run concurrently: with KST_DEVICE_SYNC=0, AMD_SERIALIZE_KERNEL or
+ * HIP_LAUNCH_BLOCKING set in the environment when the backend is created,
+ * events order them instead (profilers that collect counters serialize
+ * kernels: run them with KST_DEVICE_SYNC=0). A wait that times out (20 s)
+ * fails the run with KST_EDEVICE. */
 int kst_run_staged(kst_backend *b, int use_graph);
 /* Copies out the staged run's output blocks (nblocks * block_frames * 2) and
  * per-block stats (nblocks entries, as stats() would read after each render). */

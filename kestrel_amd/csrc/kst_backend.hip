@@ -241,6 +241,12 @@ struct kst_backend {
     hipStream_t side = nullptr;
     hipEvent_t ev_rendered = nullptr, ev_selected = nullptr;
     uint32_t sync_seq = 0;  // last sequence number of a device-ordered block
+    // Staged runs order their two streams on the device (see launch_block)
+    // unless kernels may be serialized across streams, where a device-side
+    // wait could not be met: KST_DEVICE_SYNC=0, AMD_SERIALIZE_KERNEL or
+    // HIP_LAUNCH_BLOCKING set (profilers that collect counters serialize too;
+    // run them with KST_DEVICE_SYNC=0). Events order the streams then.
+    bool device_sync = true;
 
     ~kst_backend() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -764,6 +770,15 @@ int kst_create(const kst_config *cfg, const int16_t *pool, size_t pool_len,
             return cleanup(fail(KST_EDEVICE, "creating timing events"));
     b->profile = c.profile <= KST_PROFILE_RENDER_SAMPLED ? c.profile : KST_PROFILE_ALL;
     if (b->profile == 0 && getenv("KST_PROFILE") != nullptr) b->profile = KST_PROFILE_ALL;
+    {
+        auto set = [](const char *name) {
+            const char *v = getenv(name);
+            return v != nullptr && *v != '\0' && std::strcmp(v, "0") != 0;
+        };
+        const char *ds = getenv("KST_DEVICE_SYNC");
+        b->device_sync = !(ds != nullptr && std::strcmp(ds, "0") == 0) &&
+                         !set("AMD_SERIALIZE_KERNEL") && !set("HIP_LAUNCH_BLOCKING");
+    }
     *out = b;
     return KST_OK;
 }
@@ -1083,7 +1098,7 @@ int kst_run_staged(kst_backend *b, int use_graph) {
         for (uint32_t i = 0; i < b->staged; ++i)
             b->launch_block(b->cfg.max_voices, n, true, true, chan,
                             b->profile && i % stride == 0 ? &b->stage_ev[6 * i] : nullptr, pipe,
-                            true);
+                            b->device_sync);
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipStreamSynchronize(s));

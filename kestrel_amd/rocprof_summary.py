@@ -1,9 +1,9 @@
 """Summaries of rocprofv3 CSV output for the render path's profiles.
 
-    python -m kestrel_amd.rocprof_summary trace DIR [--blocks K]
-        per-kernel mean duration over the last K blocks of a --kernel-trace run
-        (blocks start at each k_spawn dispatch, else at each k_begin), plus the
-        whole-run --stats table;
+    python -m kestrel_amd.rocprof_summary trace DIR [--blocks K] [--skip-last S]
+        per-kernel mean duration over the K blocks before the last S of a
+        --kernel-trace run (blocks start at each k_spawn dispatch, else at each
+        k_begin), plus the whole-run --stats table;
     python -m kestrel_amd.rocprof_summary pmc DIR --kernel NAME [--last K]
         mean of each counter over the last K dispatches of one kernel;
     python -m kestrel_amd.rocprof_summary traffic --fetch DIR --write DIR
@@ -47,7 +47,12 @@ def _rows(path: str):
         return list(csv.DictReader(f))
 
 
-def trace(d: str, blocks: int) -> str:
+# Kernels that wait on the device for another stream's work: their duration
+# is mostly that wait.
+WAITERS = {"k_sel_begin"}
+
+
+def trace(d: str, blocks: int, skip: int = 0) -> str:
     rows = _rows(_one(d, "kernel_trace.csv"))
     rows.sort(key=lambda r: int(r["Start_Timestamp"]))
     # A block's chain opens with k_spawn (staged runs begin each block inside
@@ -55,10 +60,11 @@ def trace(d: str, blocks: int) -> str:
     begins = [i for i, r in enumerate(rows) if short(r["Kernel_Name"]) == "k_spawn"]
     if len(begins) < blocks:
         begins = [i for i, r in enumerate(rows) if short(r["Kernel_Name"]) == "k_begin"]
-    out = [f"### Last {blocks} blocks (from {os.path.basename(_one(d, 'kernel_trace.csv'))})", "",
+    where = f"last {blocks}" if not skip else f"{blocks} before the last {skip}"
+    out = [f"### The {where} blocks (from {os.path.basename(_one(d, 'kernel_trace.csv'))})", "",
            "| kernel | launches | mean us | share of block |", "|---|---:|---:|---:|"]
-    if len(begins) >= blocks:
-        seg = rows[begins[-blocks]:]
+    if len(begins) >= blocks + skip:
+        seg = rows[begins[-blocks - skip]:(begins[-skip] if skip else len(rows))]
         agg = collections.OrderedDict()
         for r in seg:
             name = short(r["Kernel_Name"])
@@ -66,13 +72,15 @@ def trace(d: str, blocks: int) -> str:
             a = agg.setdefault(name, [0, 0.0])
             a[0] += 1
             a[1] += dur
-        t0 = int(seg[0]["Start_Timestamp"])
-        t1 = max(int(r["End_Timestamp"]) for r in seg)
-        span = (t1 - t0) / 1e3 / blocks
-        busy = sum(v[1] for v in agg.values()) / blocks
+        # Block period: from the first block's k_spawn to the last one's (the
+        # time after the last block runs into whatever the host does next).
+        first = int(rows[begins[-blocks - skip]]["Start_Timestamp"])
+        last = int(rows[begins[-skip - 1]]["Start_Timestamp"])
+        span = (last - first) / 1e3 / max(blocks - 1, 1)
         for name, (n, tot) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
-            out.append(f"| {name} | {n} | {tot / n:.1f} | {tot / blocks / span:.1%} |")
-        out += ["", f"Wall span per block {span:.1f} us; kernel-busy {busy:.1f} us per block."]
+            note = " (spins until the render has ended)" if name in WAITERS else ""
+            out.append(f"| {name}{note} | {n} | {tot / n:.1f} | {tot / blocks / span:.1%} |")
+        out += ["", f"Block period {span:.1f} us (k_spawn to k_spawn)."]
     stats = _rows(_one(d, "kernel_stats.csv"))
     out += ["", "### Whole run (--stats)", "", "| kernel | calls | avg us | total % |",
             "|---|---:|---:|---:|"]
@@ -115,6 +123,8 @@ def main(argv=None):
     t = sub.add_parser("trace")
     t.add_argument("dir")
     t.add_argument("--blocks", type=int, default=10)
+    t.add_argument("--skip-last", type=int, default=0,
+                   help="leave out the last S blocks (e.g. bench.py's per-pass replay)")
     p = sub.add_parser("pmc")
     p.add_argument("dir")
     p.add_argument("--kernel", default="k_render_fast")
@@ -129,7 +139,7 @@ def main(argv=None):
                     help="the profiled run's mean voices per block, recorded in the JSON")
     a = ap.parse_args(argv)
     if a.cmd == "trace":
-        print(trace(a.dir, a.blocks))
+        print(trace(a.dir, a.blocks, a.skip_last))
     elif a.cmd == "pmc":
         m, n, name = pmc_means(a.dir, a.kernel, a.last)
         print(f"{name}: mean over the last {n} dispatches")

@@ -193,12 +193,15 @@ def test_staged_run_equals_per_block_calls():
         b.close()
 
 
-def test_pipelined_staged_run_equals_per_block_calls_while_stealing():
+@pytest.mark.parametrize("device_sync", ["1", "0"])
+def test_pipelined_staged_run_equals_per_block_calls_while_stealing(device_sync, monkeypatch):
     """A staged run of a pool large enough to select each block's victims on
-    the side stream during the block before (with and without the graph)
+    the side stream during the block before (with and without the graph; the
+    two streams ordered on the device, or by events with KST_DEVICE_SYNC=0)
     gives the per-block calls' outputs, stats and pool bit for bit, over
     blocks that steal."""
     from kestrel_amd.backend import HipSynth
+    monkeypatch.setenv("KST_DEVICE_SYNC", device_sync)
     cfg = W.config_c2(max_voices=40_000)
     bank = W.make_bank(11, instruments=16, zones=4, key_lo=0, key_hi=127)
     notes = W.burst_notes(12, 0.6, base_rate_hz=150_000.0, burst_notes_n=20_000,
