@@ -732,7 +732,9 @@ int kst_create(const kst_config *cfg, const int16_t *pool, size_t pool_len,
         b->st.alloc(1) != hipSuccess)
         return cleanup(fail(KST_EDEVICE, "allocating the select/sort scratch"));
     if (b->fast) {
-        b->order_stride = (b->max_rows + XCDS - 1) / XCDS + 2;
+        // An XCD's list: at most ceil(rows / XCDS) + 7 survivor batches (its
+        // chunks, rounded up) and its share of the new voices' batches.
+        b->order_stride = (b->max_rows + XCDS - 1) / XCDS + RUN_CHUNKS / XCDS + 2;
         if (b->batch_victim.alloc(b->max_rows + 1) != hipSuccess ||
             b->render_order.alloc((size_t)XCDS * b->order_stride) != hipSuccess)
             return cleanup(fail(KST_EDEVICE, "allocating the render's dispatch lists"));

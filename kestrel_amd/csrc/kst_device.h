@@ -103,6 +103,29 @@ __device__ __forceinline__ uint32_t xcd_group_rev(uint32_t id, uint32_t n) {
     return x * q + (x < r ? x : r) + (cnt - 1 - k);
 }
 
+// The render's dispatch lists deal the sorted pool's survivor batches to the
+// XCDs in RUN_CHUNKS interleaved chunks (chunk c to XCD c mod XCDS): each XCD
+// renders several stretches of the pool, so no XCD draws a region whose voices
+// cost more than the rest. run_len: how many survivor batches XCD x renders;
+// run_batch: its i-th in pool order.
+constexpr uint32_t RUN_CHUNKS = 8 * XCDS;
+__device__ __forceinline__ uint32_t run_chunk(uint32_t nsurv) {
+    const uint32_t ch = (nsurv + RUN_CHUNKS - 1) / RUN_CHUNKS;
+    return ch > 0 ? ch : 1u;
+}
+__device__ __forceinline__ uint32_t run_len(uint32_t nsurv, uint32_t x) {
+    const uint32_t ch = run_chunk(nsurv);
+    const uint32_t chunks = (nsurv + ch - 1) / ch;  // the last may be short
+    if (x >= chunks) return 0;
+    const uint32_t mine = (chunks - 1 - x) / XCDS + 1;  // chunks x, x + XCDS, ...
+    const uint32_t lastc = x + (mine - 1) * XCDS;
+    return (mine - 1) * ch + (nsurv - lastc * ch < ch ? nsurv - lastc * ch : ch);
+}
+__device__ __forceinline__ uint32_t run_batch(uint32_t nsurv, uint32_t x, uint32_t i) {
+    const uint32_t ch = run_chunk(nsurv);
+    return (x + (i / ch) * XCDS) * ch + i % ch;
+}
+
 struct DevState {
     uint32_t live;    // compacted live count at the start of the block
     uint32_t parity;  // which pool buffer holds the live pool

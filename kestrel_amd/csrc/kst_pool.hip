@@ -1131,8 +1131,8 @@ __global__ void k_end(Args a) {
 }
 
 // The per-block render's dispatch lists (one workgroup per XCD). The live
-// pool's survivor batches [0, nsurv) are split into XCDS contiguous runs, as
-// xcd_group splits them; the batches [nsurv, nb) that hold new voices are dealt
+// pool's survivor batches [0, nsurv) are dealt to the XCDs in interleaved
+// chunks (run_batch); the batches [nsurv, nb) that hold new voices are dealt
 // to the XCDs in turn. XCD x renders, in order: its new-voice batches, its
 // run's batches holding a steal victim, then its run's other batches, each
 // class walked from the run's end. New voices start late and victims fade, so
@@ -1146,9 +1146,7 @@ __global__ void __launch_bounds__(1024) k_order(Args a) {
     const uint32_t nsurv = st->live / 64 < nb ? st->live / 64 : nb;
     const uint32_t ns = nb - nsurv;
     const uint32_t x = blockIdx.x;
-    const uint32_t q = nsurv / XCDS, r = nsurv % XCDS;
-    const uint32_t n = q + (x < r ? 1u : 0u);
-    const uint32_t r1 = x * q + (x < r ? x : r) + n;  // the run is [r1 - n, r1)
+    const uint32_t n = run_len(nsurv, x);  // its run: run_batch(nsurv, x, 0 .. n - 1)
     const uint32_t nsp = ns > x ? (ns - 1 - x) / XCDS + 1 : 0;
     uint32_t *out = a.render_order + (size_t)x * a.order_stride;
     const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6, nw = blockDim.x >> 6;
@@ -1156,7 +1154,7 @@ __global__ void __launch_bounds__(1024) k_order(Args a) {
     __shared__ uint32_t wsum[16], total;
     // How many of the run's batches hold a victim.
     uint32_t cnt = 0;
-    for (uint32_t j = t; j < n; j += blockDim.x) cnt += a.batch_victim[r1 - 1 - j] ? 1u : 0u;
+    for (uint32_t j = t; j < n; j += blockDim.x) cnt += a.batch_victim[run_batch(nsurv, x, j)] ? 1u : 0u;
     for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
     if (lane == 0) wsum[wave] = cnt;
     __syncthreads();
@@ -1172,7 +1170,8 @@ __global__ void __launch_bounds__(1024) k_order(Args a) {
     for (uint32_t c0 = 0; c0 < n; c0 += blockDim.x) {
         const uint32_t j = c0 + t;
         const bool in = j < n;
-        const bool e = in && a.batch_victim[r1 - 1 - j] != 0;
+        const uint32_t bj = in ? run_batch(nsurv, x, n - 1 - j) : 0u;
+        const bool e = in && a.batch_victim[bj] != 0;
         const uint64_t m = __ballot(e);
         const uint32_t below = __popcll(m & ((1ull << lane) - 1ull));
         __syncthreads();  // wsum of the previous chunk is consumed
@@ -1186,7 +1185,7 @@ __global__ void __launch_bounds__(1024) k_order(Args a) {
         if (in) {
             const uint32_t pe = before_e + wbase + below;  // victims before this one
             const uint32_t pos = e ? nsp + pe : nsp + nvict + (j - pe);
-            out[pos] = r1 - 1 - j;
+            out[pos] = bj;
         }
         before_e += ctotal;
     }

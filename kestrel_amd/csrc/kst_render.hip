@@ -1073,8 +1073,7 @@ k_render_fast(Args a) {
         const uint32_t ns = nb - nsurv;
         const uint32_t x = blockIdx.x % XCDS;
         const uint32_t k = (blockIdx.x / XCDS) * wpg + wave;
-        const uint32_t len = nsurv / XCDS + (x < nsurv % XCDS ? 1u : 0u) +
-                             (ns > x ? (ns - 1 - x) / XCDS + 1 : 0u);
+        const uint32_t len = run_len(nsurv, x) + (ns > x ? (ns - 1 - x) / XCDS + 1 : 0u);
         if (k >= len) return;
         batch = a.render_order[(size_t)x * a.order_stride + k];
     } else {
@@ -1349,8 +1348,8 @@ void launch_fast_t(const Args &a, hipStream_t s, hipEvent_t start, hipEvent_t st
     uint32_t grid = a.render_groups;
     if (a.render_order) {
         // Each XCD's list holds at most its run plus its share of the new
-        // voices' batches: ceil(batches / 8) + 2 entries.
-        const uint32_t per_xcd = (a.render_groups * wpg + XCDS - 1) / XCDS + 2;
+        // voices' batches: ceil(batches / 8) + 9 entries (its chunks round up).
+        const uint32_t per_xcd = (a.render_groups * wpg + XCDS - 1) / XCDS + RUN_CHUNKS / XCDS + 2;
         grid = XCDS * ((per_xcd + wpg - 1) / wpg);
     }
     if (start || stop)
