@@ -95,6 +95,22 @@ def test_c1_other_interpolations(interp):
     run_stream(cfg, bank, W.BlockBuilder(cfg, bank, notes), 60, min_live=100)
 
 
+@pytest.mark.parametrize("interp", [0, 1, 2])
+def test_bank_holding_minus_32768_takes_the_clamped_taps(interp):
+    """-32768 is the one sample value the -1 clamp of src/cpu.rs:190-194
+    changes; a bank holding it runs the render's clamped tap variants."""
+    import dataclasses
+    cfg, bank, notes = W.workload_c1()
+    pool = bank.pool.copy()
+    for s0, n in zip(bank.smp_start, bank.smp_len):
+        pool[int(s0) + 5:int(s0) + int(n):97] = -32768
+    assert (pool == -32768).sum() > 100
+    bank = dataclasses.replace(bank, pool=pool)
+    cfg = cfg.replace(interpolation=interp)
+    print(f"clamped taps interp={interp}", flush=True)
+    run_stream(cfg, bank, W.BlockBuilder(cfg, bank, notes), 40, min_live=100)
+
+
 @pytest.mark.parametrize("rule,admit", [(2, 0), (0, 0), (1, 1), (2, 1)])
 def test_saturated_small_pool_steals_and_drops(rule, admit):
     """Pool of 512 under ~3000 note-ons a second: steal/drop every block."""
