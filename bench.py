@@ -30,7 +30,9 @@ Reported beside it:
   block on one thread (the reference's CPU backend is single-threaded);
 * roofline: the render kernel's algorithmic bytes per launch over its mean
   duration (HIP events on the backend's own stream), against 8 TB/s, with the
-  HBM traffic of a counter profile of the same workload when one is committed;
+  HBM traffic of a counter profile of the same workload when one is committed,
+  and from its SQ counter profile the VALU instructions per voice-frame and the
+  share of the chip's VALU issue slots they fill in this run's render time;
 * precision: the device's and the CPU f32 render's error against an FP64 render
   of the same algorithm (f64 design, f64 voice arithmetic, f64 mix).
 """
@@ -668,6 +670,31 @@ def measured_traffic(wl: str, mean_voices: float):
         f"(mean voices {mv:.0f})"
 
 
+# MI355X VALU issue: 256 CUs x 4 SIMDs, one wave64 instruction per SIMD every
+# 4 cycles at 2.4 GHz.
+VALU_WAVE_INSTS_PER_S = 1024 * 2.4e9 / 4
+
+
+def measured_valu(wl: str, mean_voices: float, render_ms: float):
+    """(VALU instructions per voice-frame, VALU issue fraction, source) of the
+    render from the committed SQ counter profile of this workload, when its mean
+    voice count is within 2 % of this run's: the profile's instruction count,
+    scaled to this run's voices, over this run's render time at one wave64
+    instruction per SIMD every 4 cycles. Else Nones."""
+    path = os.path.join(ROOT, "profiles", f"render_valu_{wl}.json")
+    if not os.path.exists(path) or not render_ms:
+        return None, None, None
+    with open(path) as f:
+        d = json.load(f)
+    mv = float(d.get("mean_voices") or 0)
+    if d.get("workload") != wl or not mv or abs(mv - mean_voices) > 0.02 * mean_voices:
+        return None, None, None
+    insts = float(d["sq_insts_valu_per_launch"]) * mean_voices / mv
+    frac = insts / (VALU_WAVE_INSTS_PER_S * render_ms * 1e-3)
+    return d.get("valu_per_voice_frame"), round(frac, 4), \
+        f"profiles/render_valu_{wl}.json (mean voices {mv:.0f})"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -799,9 +826,12 @@ def main():
     if render_ms:
         achieved = float(bytes_block.mean()) / (render_ms * 1e-3) / 1e9
         traffic, source = measured_traffic(wl, float(V.mean()))
+        vpf, vfrac, vsource = measured_valu(wl, float(V.mean()), render_ms)
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
                     "traffic": traffic, "traffic_source": source, "kernel": "k_render_fast",
+                    "valu_per_voice_frame": vpf, "valu_issue_frac": vfrac,
+                    "valu_source": vsource,
                     "algorithmic_bytes_per_launch": round(float(bytes_block.mean())),
                     "bytes_per_voice_sample": round(per_unit, 4),
                     "render_ms_per_block": round(render_ms, 4),

@@ -11,6 +11,11 @@
         HBM bytes per launch of NAME: FETCH_SIZE and WRITE_SIZE (KiB per
         dispatch) each scaled by bytes-known / counter-read for the plain 1 GiB
         copy of ``bench.py --calibrate-copy``; prints markdown and a JSON line.
+    python -m kestrel_amd.rocprof_summary valu DIR --frames F --mean-voices V
+            [--kernel NAME] [--last K] [--workload W]
+        SQ_INSTS_VALU of NAME per launch and per voice-frame (a wave64
+        instruction serves the 64 voices of one batch, so per voice-frame it is
+        the count x 64 / (V x F)); prints a JSON line.
 
 DIR is the ``-d`` directory of a ``rocprofv3 ... --output-format csv`` run.
 """
@@ -137,8 +142,25 @@ def main(argv=None):
     tr.add_argument("--workload", default=None, help="workload name recorded in the JSON")
     tr.add_argument("--mean-voices", type=float, default=None,
                     help="the profiled run's mean voices per block, recorded in the JSON")
+    va = sub.add_parser("valu")
+    va.add_argument("dir")
+    va.add_argument("--kernel", default="k_render_fast")
+    va.add_argument("--last", type=int, default=10)
+    va.add_argument("--frames", type=int, required=True, help="frames per block")
+    va.add_argument("--mean-voices", type=float, required=True,
+                    help="the profiled run's mean voices per block")
+    va.add_argument("--workload", default=None)
     a = ap.parse_args(argv)
-    if a.cmd == "trace":
+    if a.cmd == "valu":
+        m, n, name = pmc_means(a.dir, a.kernel, a.last)
+        insts = m["SQ_INSTS_VALU"]
+        rec = {"kernel": name, "sq_insts_valu_per_launch": round(insts),
+               "valu_per_voice_frame": round(insts * 64.0 / (a.mean_voices * a.frames), 3),
+               "frames": a.frames, "mean_voices": a.mean_voices, "launches": n}
+        if a.workload:
+            rec["workload"] = a.workload
+        print(json.dumps(rec))
+    elif a.cmd == "trace":
         print(trace(a.dir, a.blocks, a.skip_last))
     elif a.cmd == "pmc":
         m, n, name = pmc_means(a.dir, a.kernel, a.last)
