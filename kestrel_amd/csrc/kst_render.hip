@@ -291,6 +291,21 @@ __device__ __forceinline__ uint32_t advance_index(uint32_t idx, int32_t off, boo
     return (uint32_t)r;
 }
 
+// The Catmull-Rom coefficients of src/cpu.rs:297-302,
+//   ca = -0.5 sm1 + 1.5 s0 - 1.5 s1 + 0.5 s2,
+//   cb = sm1 - 2.5 s0 + 2 s1 - 0.5 s2,
+//   cc = -0.5 sm1 + 0.5 s1,
+// each summed left to right. A term scaled by 0.5 or 2 is exact (every sample
+// is 0 or at least 1/32767 in magnitude and at most 1), so adding it is one
+// fused multiply-add with the same single rounding: the same bits in 11
+// operations instead of 16.
+__device__ __forceinline__ void cubic_coefs(float sm1, float s0, float s1, float s2, float &ca,
+                                            float &cb, float &cc) {
+    ca = fmaf(0.5f, s2, fmaf(-0.5f, sm1, 1.5f * s0) - 1.5f * s1);
+    cb = fmaf(-0.5f, s2, fmaf(2.0f, s1, sm1 - 2.5f * s0));
+    cc = fmaf(-0.5f, sm1, 0.5f * s1);
+}
+
 template <int INTERP>
 __device__ __forceinline__ float interpolate(const Args &a, const Voice &v, uint32_t idx,
                                              float frac, bool looping) {
@@ -309,9 +324,8 @@ __device__ __forceinline__ float interpolate(const Args &a, const Voice &v, uint
     const float s1 = fetch(a, v.base, i1);
     const float s2 = fetch(a, v.base, i2);
     // src/cpu.rs:297-302, Horner form.
-    const float ca = -0.5f * sm1 + 1.5f * s0 - 1.5f * s1 + 0.5f * s2;
-    const float cb = sm1 - 2.5f * s0 + 2.0f * s1 - 0.5f * s2;
-    const float cc = -0.5f * sm1 + 0.5f * s1;
+    float ca, cb, cc;
+    cubic_coefs(sm1, s0, s1, s2, ca, cb, cc);
     return ((ca * frac + cb) * frac + cc) * frac + s0;
 }
 
@@ -703,9 +717,8 @@ __device__ __forceinline__ void fast_subtile(const Args &a, Voice &v, float *tb,
             const float s1 = norm_lo<CLAMP>(w1[i]);
             const float s2 = norm_hi<CLAMP>(w1[i]);
             const float frac = flo * FRAC_SCALE;
-            const float ca = -0.5f * sm1 + 1.5f * s0 - 1.5f * s1 + 0.5f * s2;
-            const float cb = sm1 - 2.5f * s0 + 2.0f * s1 - 0.5f * s2;
-            const float cc = -0.5f * sm1 + 0.5f * s1;
+            float ca, cb, cc;
+            cubic_coefs(sm1, s0, s1, s2, ca, cb, cc);
             s = ((ca * frac + cb) * frac + cc) * frac + s0;
         }
         const float x = s * g;
