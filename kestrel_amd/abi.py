@@ -180,8 +180,7 @@ def load() -> C.CDLL:
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
-            raise BackendError(f"{LIB_PATH} is not built; run kestrel_amd/build.py "
-                               "(or __graft_entry__.build())")
+            raise BackendError(f"{LIB_PATH} is not built; run python kestrel_amd/build.py")
         lib = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(lib, name)
