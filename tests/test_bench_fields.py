@@ -46,3 +46,11 @@ def test_profiles_of_another_workload_or_size_give_nothing():
     assert bench.measured_traffic("c2", mt)[0] == _profile("render_traffic_c2.json")["bytes_per_launch"]
     assert bench.measured_traffic("c2", mt * 1.05) == (None, None)
     assert bench.measured_traffic("c1", mt) == (None, None)
+
+
+def test_more_than_one_gpu_exits_with_a_message():
+    import subprocess
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "measures one GPU" in r.stdout
