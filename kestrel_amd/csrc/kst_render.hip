@@ -716,10 +716,14 @@ __device__ __forceinline__ void fast_subtile(const Args &a, Voice &v, float *tb,
             const float s0 = norm_hi<CLAMP>(w0[i]);
             const float s1 = norm_lo<CLAMP>(w1[i]);
             const float s2 = norm_hi<CLAMP>(w1[i]);
-            const float frac = flo * FRAC_SCALE;
             float ca, cb, cc;
             cubic_coefs(sm1, s0, s1, s2, ca, cb, cc);
-            s = ((ca * frac + cb) * frac + cc) * frac + s0;
+            // ((ca * frac + cb) * frac + cc) * frac + s0 with frac = flo * 2^-32:
+            // each product by frac is the product by flo scaled by 2^-32, which
+            // commutes with rounding, so each step is one fused multiply-add of
+            // that product.
+            s = fmaf(fmaf(fmaf(ca * flo, FRAC_SCALE, cb) * flo, FRAC_SCALE, cc) * flo, FRAC_SCALE,
+                     s0);
         }
         const float x = s * g;
         float y = x;
