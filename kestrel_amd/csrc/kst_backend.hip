@@ -174,7 +174,7 @@ struct kst_backend {
     uint32_t slots = 0, ppv = 0, tiles = 0, gate_words = 0, chan_words = 0;
     uint32_t max_steal = 0;
     SortPlan sort{};
-    uint32_t sort_passes = 1, digit_bits = 1, tiles_max = 1;
+    uint32_t sort_passes = 1, digit_bits = 1, tiles_max = 1, scan_groups = 0;
     uint32_t dead_digit = ~0u;  // Args::dead_digit
     uint32_t max_groups = 1;  // render workgroups at capacity (plain: partial-buffer width)
     uint32_t max_rows = 1;    // wave64 partial rows at capacity (one per 64-voice batch)
@@ -186,7 +186,7 @@ struct kst_backend {
     DevBuf<kst_region_params> params;
     DevBuf<float> partials, rsum, fade_tab;
     uint32_t fade_lds = 0;
-    DevBuf<uint32_t> keys[2], idx[2], sort_hist, sort_tot, sel_cnt, cand_slot;
+    DevBuf<uint32_t> keys[2], idx[2], sort_hist, sort_tot, scan_grp, sel_cnt, cand_slot;
     // The per-block render's dispatch lists (wave64 form): per-batch victim
     // flags and one list per XCD of order_stride entries.
     DevBuf<uint32_t> batch_victim, render_order;
@@ -264,6 +264,7 @@ struct kst_backend {
         for (auto &k : idx) k.release();
         sort_hist.release();
         sort_tot.release();
+        scan_grp.release();
         sel_cnt.release();
         sel_min.release();
         sel_max.release();
@@ -404,6 +405,8 @@ struct kst_backend {
         a.sort_passes = sort_passes;
         a.digit_bits = digit_bits;
         a.tiles_max = tiles_max;
+        a.scan_grp = scan_grp.p;
+        a.scan_groups = scan_groups;
         a.region_shift = sort.region_shift;
         a.stage_shift = sort.stage_shift;
         a.phase_shift = sort.phase_shift;
@@ -731,6 +734,13 @@ int kst_create(const kst_config *cfg, const int16_t *pool, size_t pool_len,
         b->cand_key.alloc(b->slots) != hipSuccess || b->cand_slot.alloc(b->slots) != hipSuccess ||
         b->st.alloc(1) != hipSuccess)
         return cleanup(fail(KST_EDEVICE, "allocating the select/sort scratch"));
+    // Many tiles: the scan runs over groups of tiles in parallel (one scan
+    // workgroup per 32 digits would walk thousands of tiles in sequence).
+    if (b->tiles_max >= SCAN_GROUPED_MIN_TILES) {
+        b->scan_groups = (b->tiles_max + SCAN_GROUP_TILES - 1) / SCAN_GROUP_TILES;
+        if (b->scan_grp.alloc((size_t)b->scan_groups * radix) != hipSuccess)
+            return cleanup(fail(KST_EDEVICE, "allocating the grouped scan's sums"));
+    }
     if (b->fast) {
         // An XCD's list: at most ceil(rows / XCDS) + 7 survivor batches (its
         // chunks, rounded up) and its share of the new voices' batches.

@@ -58,6 +58,8 @@ constexpr uint32_t SEL_PENDING = 0xFFFFFFFFu;
 constexpr uint32_t SORT_TPB = 256;
 constexpr uint32_t SORT_IPT = 8;
 constexpr uint32_t SORT_TILE = SORT_TPB * SORT_IPT;
+// The sort's scan over groups of tiles, from pools of this many tiles on.
+constexpr uint32_t SCAN_GROUP_TILES = 512, SCAN_GROUPED_MIN_TILES = 4096;
 constexpr uint32_t MAX_SORT_PASSES = 32;
 constexpr uint32_t SORT_MAX_BITS = 9;
 constexpr uint32_t SORT_MAX_RADIX = 1u << SORT_MAX_BITS;
@@ -209,6 +211,11 @@ struct Args {
     uint32_t *sort_hist;  // [passes][radix][tiles_max]
     uint32_t *sort_tot;   // [passes][radix] digit totals
     uint32_t sort_on, sort_passes, digit_bits, tiles_max;
+    // Pools of many tiles scan each digit's counts in groups of SCAN_GROUP_TILES
+    // tiles (scan_groups = the most groups, 0: one scan kernel); scan_grp holds
+    // each group's count per digit, [groups][radix].
+    uint32_t *scan_grp;
+    uint32_t scan_groups, scan_pad;
     uint32_t region_shift, stage_shift, phase_shift, phase_mask, dead_region;
     // Fused compaction (wave64 per-block and staged runs): the render reads
     // the survivors through the previous sort's permutation and writes every

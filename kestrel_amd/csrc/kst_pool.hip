@@ -823,22 +823,47 @@ __device__ __forceinline__ uint32_t block_exscan256(uint32_t v, uint32_t *wtot, 
 // Each tile's offset within a digit is the same digit's count in the tiles
 // before it; the digit's total goes to sort_tot for the scatter to place the
 // digits. A workgroup takes SCAN_DIGITS adjacent digits (a half-wave reads a
-// tile's 32 adjacent counts at once) and splits the tiles into SCAN_LANES
+// tile's 32 adjacent counts at once) and splits its tiles into SCAN_LANES
 // contiguous ranges, one per half-wave: each sums its range, the ranges' sums
 // are scanned in LDS, and each range's offsets are written from its start.
+// Its tiles are all of them, or (k_sort_scan_grouped, pools of many tiles)
+// group blockIdx.y of SCAN_GROUP_TILES, whose digit counts k_sort_scan_sums
+// has summed: the groups before it give its start.
 constexpr uint32_t SCAN_DIGITS = 32, SCAN_LANES = 32;
-__device__ __forceinline__ void sort_scan_body(const Args &a, uint32_t p) {
+
+// Tiles [g0, g1) of this workgroup: all, or its group.
+__device__ __forceinline__ void scan_group_range(bool grouped, uint32_t tiles, uint32_t &g0,
+                                                 uint32_t &g1) {
+    if (!grouped) {
+        g0 = 0;
+        g1 = tiles;
+        return;
+    }
+    g0 = blockIdx.y * SCAN_GROUP_TILES < tiles ? blockIdx.y * SCAN_GROUP_TILES : tiles;
+    g1 = g0 + SCAN_GROUP_TILES < tiles ? g0 + SCAN_GROUP_TILES : tiles;
+}
+
+// Groups holding a tile (at least one, so some group writes the totals).
+__device__ __forceinline__ uint32_t scan_groups_used(uint32_t tiles) {
+    const uint32_t g = (tiles + SCAN_GROUP_TILES - 1) / SCAN_GROUP_TILES;
+    return g > 0 ? g : 1u;
+}
+
+// Grouped scan, first kernel: each group's count of each digit.
+__global__ void __launch_bounds__(1024) k_sort_scan_sums(Args a, uint32_t p) {
     const uint32_t radix = 1u << a.digit_bits;
     if (p >= a.sort_passes || blockIdx.x * SCAN_DIGITS >= radix) return;
-    DevState *st = a.st;
-    const uint32_t n = st->live_total;
+    const uint32_t n = a.st->live_total;
     const uint32_t tiles = (n + SORT_TILE - 1) / SORT_TILE;
+    if (blockIdx.y >= scan_groups_used(tiles)) return;
+    uint32_t g0, g1;
+    scan_group_range(true, tiles, g0, g1);
     const uint32_t col = threadIdx.x % SCAN_DIGITS, w = threadIdx.x / SCAN_DIGITS;
     const uint32_t d = blockIdx.x * SCAN_DIGITS + col;
-    const uint32_t per = (tiles + SCAN_LANES - 1) / SCAN_LANES;
-    const uint32_t t0 = w * per < tiles ? w * per : tiles;
-    const uint32_t t1 = t0 + per < tiles ? t0 + per : tiles;
-    uint32_t *h = a.sort_hist + (size_t)p * a.tiles_max * radix;
+    const uint32_t per = (g1 - g0 + SCAN_LANES - 1) / SCAN_LANES;
+    const uint32_t t0 = g0 + w * per < g1 ? g0 + w * per : g1;
+    const uint32_t t1 = t0 + per < g1 ? t0 + per : g1;
+    const uint32_t *h = a.sort_hist + (size_t)p * a.tiles_max * radix;
     __shared__ uint32_t part[SCAN_LANES][SCAN_DIGITS];
     uint32_t sum = 0;
     if (d < radix) {
@@ -847,7 +872,50 @@ __device__ __forceinline__ void sort_scan_body(const Args &a, uint32_t p) {
     }
     part[w][col] = sum;
     __syncthreads();
+    if (w == 0 && d < radix) {
+        uint32_t tot = 0;
+        for (uint32_t l = 0; l < SCAN_LANES; ++l) tot += part[l][col];
+        a.scan_grp[(size_t)blockIdx.y * radix + d] = tot;
+    }
+}
+
+template <bool GROUPED>
+__device__ __forceinline__ void sort_scan_body(const Args &a, uint32_t p) {
+    const uint32_t radix = 1u << a.digit_bits;
+    if (p >= a.sort_passes || blockIdx.x * SCAN_DIGITS >= radix) return;
+    DevState *st = a.st;
+    const uint32_t n = st->live_total;
+    const uint32_t tiles = (n + SORT_TILE - 1) / SORT_TILE;
+    const uint32_t used = GROUPED ? scan_groups_used(tiles) : 1u;
+    if (blockIdx.y >= used) return;
+    uint32_t g0, g1;
+    scan_group_range(GROUPED, tiles, g0, g1);
+    const uint32_t col = threadIdx.x % SCAN_DIGITS, w = threadIdx.x / SCAN_DIGITS;
+    const uint32_t d = blockIdx.x * SCAN_DIGITS + col;
+    const uint32_t per = (g1 - g0 + SCAN_LANES - 1) / SCAN_LANES;
+    const uint32_t t0 = g0 + w * per < g1 ? g0 + w * per : g1;
+    const uint32_t t1 = t0 + per < g1 ? t0 + per : g1;
+    uint32_t *h = a.sort_hist + (size_t)p * a.tiles_max * radix;
+    __shared__ uint32_t part[SCAN_LANES][SCAN_DIGITS];
+    __shared__ uint32_t before[GROUPED ? SCAN_LANES : 1][SCAN_DIGITS];
+    if (GROUPED) {
+        // The groups before this one: lane w sums groups w, w + 32, ...
+        uint32_t pre = 0;
+        if (d < radix)
+            for (uint32_t q = w; q < blockIdx.y; q += SCAN_LANES)
+                pre += a.scan_grp[(size_t)q * radix + d];
+        before[GROUPED ? w : 0][col] = pre;
+    }
+    uint32_t sum = 0;
+    if (d < radix) {
+#pragma unroll 8
+        for (uint32_t t = t0; t < t1; ++t) sum += h[(size_t)t * radix + d];
+    }
+    part[w][col] = sum;
+    __syncthreads();
     uint32_t run = 0;
+    if (GROUPED)
+        for (uint32_t l = 0; l < SCAN_LANES; ++l) run += before[GROUPED ? l : 0][col];
     for (uint32_t l = 0; l < w; ++l) run += part[l][col];
     if (d < radix) {
 #pragma unroll 8
@@ -856,13 +924,18 @@ __device__ __forceinline__ void sort_scan_body(const Args &a, uint32_t p) {
             h[(size_t)t * radix + d] = run;
             run += v;
         }
-        if (w == SCAN_LANES - 1) a.sort_tot[p * radix + d] = run;
+        if (w == SCAN_LANES - 1 && blockIdx.y == used - 1) a.sort_tot[p * radix + d] = run;
     }
 }
-__global__ void __launch_bounds__(1024) k_sort_scan(Args a, uint32_t p) { sort_scan_body(a, p); }
+__global__ void __launch_bounds__(1024) k_sort_scan(Args a, uint32_t p) {
+    sort_scan_body<false>(a, p);
+}
+__global__ void __launch_bounds__(1024) k_sort_scan_grouped(Args a, uint32_t p) {
+    sort_scan_body<true>(a, p);
+}
 // The same pass over a batch of renders: blockIdx.z picks the render.
 __global__ void __launch_bounds__(1024) kb_sort_scan(const Args *__restrict__ aa, uint32_t p) {
-    sort_scan_body(aa[blockIdx.z], p);
+    sort_scan_body<false>(aa[blockIdx.z], p);
 }
 
 // Stable scatter of one digit pass. Each wave ranks its own contiguous quarter
@@ -1236,7 +1309,13 @@ void launch_sort(const Args &a, uint32_t grid, uint32_t gather_grid, hipStream_t
     const uint32_t radix = 1u << a.digit_bits;
     for (uint32_t p = 0; p < a.sort_passes; ++p) {
         k_sort_hist<<<grid, SORT_TPB, 0, s>>>(a, p);
-        k_sort_scan<<<(radix + SCAN_DIGITS - 1) / SCAN_DIGITS, SCAN_DIGITS * SCAN_LANES, 0, s>>>(a, p);
+        const dim3 sg((radix + SCAN_DIGITS - 1) / SCAN_DIGITS, a.scan_groups ? a.scan_groups : 1);
+        if (a.scan_groups) {
+            k_sort_scan_sums<<<sg, SCAN_DIGITS * SCAN_LANES, 0, s>>>(a, p);
+            k_sort_scan_grouped<<<sg, SCAN_DIGITS * SCAN_LANES, 0, s>>>(a, p);
+        } else {
+            k_sort_scan<<<sg, SCAN_DIGITS * SCAN_LANES, 0, s>>>(a, p);
+        }
         k_sort_scatter<<<grid, SORT_TPB, 0, s>>>(a, p);
     }
     if (!a.fused) k_gather<<<gather_grid, 256, 0, s>>>(a);
