@@ -480,9 +480,13 @@ def run_c5(args):
     roofline = None
     if timings.get("render"):
         achieved = float(bytes_block.mean()) / (timings["render"] * 1e-3) / 1e9
+        traffic, source = measured_traffic("c5", float(V.mean()))
+        vpf, vfrac, vsource = measured_valu("c5", float(V.mean()), timings["render"])
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
-                    "kernel": "k_render_fast",
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                    "traffic": traffic, "traffic_source": source,
+                    "valu_per_voice_frame": vpf, "valu_issue_frac": vfrac,
+                    "valu_source": vsource, "kernel": "k_render_fast",
                     "algorithmic_bytes_per_launch": round(float(bytes_block.mean())),
                     "bytes_per_voice_sample": round(float(bytes_block.sum() / vs), 4),
                     "render_ms_per_block": round(timings["render"], 4)}

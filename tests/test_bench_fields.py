@@ -16,10 +16,11 @@ def _profile(name):
 
 
 def test_committed_profiles_name_their_workload_and_voices():
-    for name in ("render_traffic_c2.json", "render_valu_c2.json"):
-        d = _profile(name)
-        assert d["workload"] == "c2" and d["mean_voices"] > 1_000_000
-        assert d["kernel"] == "k_render_fast" and "command" in d
+    for wl, voices in (("c2", 1_000_000), ("c5", 60_000_000)):
+        for kind in ("traffic", "valu"):
+            d = _profile(f"render_{kind}_{wl}.json")
+            assert d["workload"] == wl and d["mean_voices"] > voices
+            assert d["kernel"] == "k_render_fast" and "command" in d
 
 
 def test_valu_fields_follow_the_profile_and_this_runs_render_time():
